@@ -1,0 +1,160 @@
+"""Benchmark of the channel-dns per-step hot path on one MI355X.
+
+A step is one full semi-implicit time step (nonlinear term through the 3/2-dealiased
+product grid, RHS, coupled tau solves, mean-mode and flow corrections) of the
+Re_tau=180 channel: base grid 128x129x128, product grid 196x193x196, FP64.  The state
+is resident in HBM before timing starts; steps 1-2 (solver setup) are warm-up.
+
+Prints one JSON line.  `roofline` is for the kernel with the largest share of step time:
+algorithmic bytes per launch (see `ALGO`) over its mean launch time from HIP events on
+the backend's stream.  `cpu_baseline` times the CPU oracle (a port of the reference
+algorithm, OpenMP threads) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "channel-dns_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(cfg, esz):
+    """Bytes each kernel must move once per launch (nonzero inputs + outputs, 16-byte complex
+    in FP64, esz-byte complex for physical-side buffers)."""
+    nkx, nzk, nj = cfg["NXS"] + 1, 2 * cfg["NZS"] + 1, cfg["NYS"] + 1
+    NQ, NL = nkx * nzk, cfg["NY2"] + 1
+    S = NQ * nj            # one spectral component
+    Y = NL * NQ            # one y-physical field
+    Z = NL * cfg["NZ2"] * nkx  # one z-physical field
+    zc = 16
+    return {
+        "curl": 6 * S * zc,
+        "y_backward": 6 * S * zc + 6 * Y * esz,
+        "z_backward": 6 * Y * esz + 6 * Z * esz,
+        "x_product": 6 * Z * esz + 3 * Z * esz,
+        "z_forward": 3 * Z * esz + 3 * Y * esz,
+        "y_forward": 3 * Y * esz + 3 * S * zc,
+        # u, u_prev, N, N_prev, 12 influence/B columns, 2 real ad columns in; u, P out
+        "solve_step": (12 * S + 12 * S) * zc + 2 * S * 8 + 4 * S * zc,
+    }
+
+
+def cpu_baseline(cfg, u0, seconds_target=15.0, threads=None):
+    import numpy as np  # noqa: F401
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    n = threads or min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(n)
+    o = pyoracle.Oracle(cfg)
+    o.set_state("velocity", u0)
+    o.step(1)
+    o.step(2)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.step(3 + k)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target or k >= 50:
+            break
+    return {"value": k / el, "unit": "steps/s", "cores": n, "kind": "port",
+            "sample": f"{k} steps (it=3..{2 + k}) of the same workload, oracle on {n} OpenMP threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="retau180")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs one GPU per process; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import cdns
+    from cdns import config, fields
+
+    torch.cuda.set_device(0)
+    cfg = config.WORKLOADS[args.workload]()
+    u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
+    warm = max(args.warmup, 2)
+
+    res = {}
+    for prec, name in ((cdns.FP64, "f64"),):
+        dev = cdns.Channel(cfg, precision=prec)
+        dev.set_state("velocity", u0)
+        dev.run(warm, it_begin=1)
+        dev.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.run(args.steps, it_begin=warm + 1)
+        dev.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        # per-kernel times: the same number of steps again with HIP events on the backend stream
+        dev.profile(True)
+        dev.profile_reset()
+        dev.run(args.steps, it_begin=warm + args.steps + 1)
+        dev.sync()
+        prof = dev.profile_read()
+        dev.profile(False)
+        u = dev.get_state("velocity")
+        res[name] = dict(elapsed=el, prof=prof, finite=bool(np.isfinite(u).all()))
+        dev.close()
+
+    r = res["f64"]
+    steps_per_s = args.steps / r["elapsed"]
+    algo = algorithmic_bytes(cfg, 16)
+    kern = {}
+    for k, (cnt, tot) in r["prof"].items():
+        kern[k] = {"launches": cnt, "ms_per_launch": tot / max(cnt, 1), "ms_per_step": tot / args.steps}
+        if k in algo and cnt:
+            gbs = algo[k] / (tot / cnt * 1e-3) / 1e9
+            kern[k]["achieved_GBs"] = gbs
+            kern[k]["frac_hbm"] = gbs / HBM_PEAK_GBS
+    dom = max((k for k in kern if k in algo), key=lambda k: kern[k]["ms_per_step"])
+    dk = kern[dom]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dk["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": algo[dom], "ms_per_launch": dk["ms_per_launch"]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg, u0, args.cpu_seconds)
+    line = {
+        "metric": "steps/s at 128x129x128 (product grid 196x193x196), FP64 step on one MI355X",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warm,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded log-law mean + divergence-free k^-5/3 perturbation (rms 10% of centreline)",
+        "config": {"workload": f"{args.workload}: base {cfg['NX']}x{cfg['NY'] + 1}x{cfg['NZ']}, product "
+                               f"{cfg['NX2']}x{cfg['NY2'] + 1}x{cfg['NZ2']}, Re={cfg['Re']}, dt={cfg['DT']}",
+                   "steps_timed_from_iteration": warm + 1},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+        "finite": r["finite"],
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1024 @@
+// C ABI of the gfx950 backend (include/cdns.h): context, device state, setup and the step.
+#include "../../include/cdns.h"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+
+using namespace cdns;
+
+namespace {
+
+// ---------------------------------------------------------------- tables --
+FftPlan make_plan(int n) {
+    FftPlan p{};
+    p.n = n;
+    int m = n, a = 0;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    auto push = [&](int r) {
+        if (p.nst >= MAXST) throw std::runtime_error("FFT plan too long");
+        p.rad[p.nst++] = r;
+    };
+    // powers of two as radix-8/4 stages (a lone 2 only when a == 1)
+    if (a == 1) push(2);
+    else if (a == 2) push(4);
+    else if (a >= 3) {
+        int r = a % 3, k8 = a / 3;
+        if (r == 1) { k8 -= 1; push(4); push(4); }
+        else if (r == 2) push(4);
+        for (int i = 0; i < k8; ++i) push(8);
+    }
+    for (int f : {3, 5, 7})
+        while (m % f == 0) { m /= f; push(f); }
+    if (m != 1) throw std::runtime_error("FFT length " + std::to_string(n) + " has a prime factor above 7");
+    return p;
+}
+
+std::vector<long double> twiddles(int n) {
+    std::vector<long double> t(2 * (size_t)n);
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; ++k) {
+        const long double a = twopi * (long double)k / (long double)n;
+        t[2 * k] = cosl(a);
+        t[2 * k + 1] = -sinl(a);
+    }
+    return t;
+}
+
+// cos(pi m / N) for m in [0, 2N), with the exact zeros at pi/2 and 3pi/2
+std::vector<long double> cospi_table(int N) {
+    std::vector<long double> t(2 * (size_t)N);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < 2 * N; ++m) {
+        long double v = cosl(pi * (long double)m / (long double)N);
+        if (2 * m == N || 2 * m == 3 * N) v = 0.0L;
+        t[m] = v;
+    }
+    return t;
+}
+
+struct Event2 {
+    hipEvent_t a, b;
+    int name;
+};
+
+} // namespace
+
+struct cdns_ctx {
+    cdns_config cfg{};
+    Geometry g{};
+    bool fp32 = false;
+    hipStream_t st = nullptr;
+    std::string err;
+    size_t esz = 16; // bytes per physical complex element
+
+    // tables
+    double *WAVX = nullptr, *WAVZ = nullptr, *WL = nullptr, *WD = nullptr, *WR = nullptr;
+    double *ycoef_b = nullptr, *ycoef_fe = nullptr, *ycoef_fo = nullptr;
+    void *twz = nullptr, *twx = nullptr;
+    FftPlan pz{}, px{};
+    int KT = 13, P = 4;
+    std::vector<double> hWAVX, hWAVZ;
+
+    // solver tables
+    double g_v = 0.0, pa = 0.0, pb = 0.0;
+    double *ad_p = nullptr, *ad_v = nullptr, *wn_p = nullptr, *wn_v = nullptr;
+    zd *PI[2] = {}, *VI[2][3] = {}, *ID[2] = {}, *PB = nullptr, *VB[3] = {}, *SIG0 = nullptr;
+    double *u_unit = nullptr, *w_unit = nullptr, *q_unit = nullptr;
+    bool setup_done = false;
+
+    // state
+    zd *u[3] = {}, *up[3] = {}, *nl[3] = {}, *nlp[3] = {}, *P_ = nullptr, *om[3] = {};
+    zd *bc_prev = nullptr, *bc_ext = nullptr, *bcw = nullptr, *vbc = nullptr;
+    double *flow = nullptr;
+    double time = 0.0;
+
+    // workspaces
+    void *Ybuf = nullptr, *Zbuf = nullptr, *ZNbuf = nullptr;
+    zd *scr[8] = {};       // spectral planes for C-ABI staging
+    zd *ref_stage = nullptr; // reference-layout spectral staging (3 components)
+    zd *bnd_stage = nullptr; // boundary staging
+    double *phys_stage = nullptr;
+
+    std::vector<void *> allocs;
+
+    // profiling
+    bool prof = false;
+    std::vector<std::string> kname;
+    std::map<std::string, int> kidx;
+    std::vector<double> ktot;
+    std::vector<long long> kcnt;
+    std::vector<Event2> pending;
+    std::vector<hipEvent_t> pool;
+
+    template <typename T> T *alloc(size_t n) {
+        void *p = nullptr;
+        CDNS_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        CDNS_HIP(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), st));
+        allocs.push_back(p);
+        return static_cast<T *>(p);
+    }
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        CDNS_HIP(hipEventCreate(&e));
+        return e;
+    }
+};
+
+namespace {
+
+// Records a HIP event pair around each named launch when profiling is on.
+struct Launch {
+    cdns_ctx *c;
+    int id = -1;
+    hipEvent_t a{}, b{};
+    Launch(cdns_ctx *ctx, const char *name) : c(ctx) {
+        if (!c->prof) return;
+        auto it = c->kidx.find(name);
+        if (it == c->kidx.end()) {
+            id = (int)c->kname.size();
+            c->kidx[name] = id;
+            c->kname.push_back(name);
+            c->ktot.push_back(0.0);
+            c->kcnt.push_back(0);
+        } else {
+            id = it->second;
+        }
+        a = c->ev();
+        b = c->ev();
+        CDNS_HIP(hipEventRecord(a, c->st));
+    }
+    ~Launch() noexcept(false) {
+        if (std::uncaught_exceptions() > 0) return;
+        CDNS_HIP(hipGetLastError());
+        if (id < 0) return;
+        CDNS_HIP(hipEventRecord(b, c->st));
+        c->pending.push_back({a, b, id});
+    }
+};
+
+void drain_profile(cdns_ctx *c) {
+    for (auto &e : c->pending) {
+        CDNS_HIP(hipEventSynchronize(e.b));
+        float ms = 0.f;
+        CDNS_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+        c->ktot[e.name] += ms;
+        c->kcnt[e.name] += 1;
+        c->pool.push_back(e.a);
+        c->pool.push_back(e.b);
+    }
+    c->pending.clear();
+}
+
+// ---------------------------------------------------------- transforms ----
+YArgs yargs(cdns_ctx *c) {
+    YArgs a{};
+    const Geometry &g = c->g;
+    a.coef = c->ycoef_b;
+    a.nj = g.nj;
+    a.nje = (g.nj + 1) / 2;
+    a.njo = g.nj / 2;
+    a.JS = g.JS;
+    a.NQ = g.NQ;
+    a.NLH = g.NLH;
+    a.NY2 = g.NY2;
+    a.YS = g.YS;
+    a.y = c->Ybuf;
+    return a;
+}
+ZArgs zargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs) {
+    ZArgs a{};
+    a.in = in;
+    a.out = out;
+    a.in_fs = in_fs;
+    a.out_fs = out_fs;
+    a.tw = c->twz;
+    a.plan = c->pz;
+    a.nkx = c->g.nkx;
+    a.nzk = c->g.nzk;
+    a.NZS = c->g.NZS;
+    a.NQ = c->g.NQ;
+    a.KT = c->KT;
+    a.NL = c->g.NL;
+    return a;
+}
+XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs, int P) {
+    XArgs a{};
+    a.in = in;
+    a.out = out;
+    a.in_fs = in_fs;
+    a.out_fs = out_fs;
+    a.tw = c->twx;
+    a.plan = c->px;
+    a.nkx = c->g.nkx;
+    a.NL = c->g.NL;
+    a.NZ2 = c->g.NZ2;
+    a.P = P;
+    return a;
+}
+
+// spectral planes -> Y fields (n fields) -> Z fields
+void backward_yz(cdns_ctx *c, const zd *const *planes, int n) {
+    YArgs a = yargs(c);
+    for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
+    {
+        Launch L(c, "y_backward");
+        launch_ybwd(a, n, c->fp32, c->st);
+    }
+    {
+        Launch L(c, "z_backward");
+        launch_zbwd(zargs(c, c->Ybuf, c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
+    }
+}
+// ZN fields (n) -> Y fields -> spectral planes
+void forward_zy(cdns_ctx *c, zd *const *planes, int n) {
+    {
+        Launch L(c, "z_forward");
+        launch_zfwd(zargs(c, c->ZNbuf, c->Ybuf, c->g.ZS, c->g.YS), n, c->fp32, c->st);
+    }
+    YArgs a = yargs(c);
+    a.coef = c->ycoef_fe;
+    a.coef2 = c->ycoef_fo;
+    for (int f = 0; f < n; ++f) a.spec_out[f] = planes[f];
+    {
+        Launch L(c, "y_forward");
+        launch_yfwd(a, n, c->fp32, c->st);
+    }
+}
+
+// N = a x b for spectral vector planes (a: 3 planes, b: 3 planes)
+void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out) {
+    const zd *f6[6] = {a[0], a[1], a[2], b[0], b[1], b[2]};
+    backward_yz(c, f6, 6);
+    {
+        Launch L(c, "x_product");
+        launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
+    }
+    forward_zy(c, out, 3);
+}
+
+void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
+    CurlArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.u[k] = u[k];
+        a.w[k] = w[k];
+    }
+    a.bcw = bcw;
+    a.WAVX = c->WAVX;
+    a.WAVZ = c->WAVZ;
+    a.Re = c->cfg.Re;
+    a.ny = c->g.NYS;
+    a.JS = c->g.JS;
+    a.NQ = c->g.NQ;
+    a.nzk = c->g.nzk;
+    Launch L(c, "curl");
+    launch_curl(a, c->g.E, c->st);
+}
+
+// TimeStepper::enqueue_nonlinear_term on the context's velocity planes
+void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
+    curl_planes(c, u, c->om, c->bcw);
+    cross_planes(c, u, c->om, nl);
+}
+
+// ---------------------------------------------------------------- solver --
+SolveArgs solve_args(cdns_ctx *c, int mode) {
+    SolveArgs a{};
+    const Geometry &g = c->g;
+    a.mode = mode;
+    a.ny = g.NYS;
+    a.JS = g.JS;
+    a.NQ = g.NQ;
+    a.nzk = g.nzk;
+    a.nkx = g.nkx;
+    a.WAVX = c->WAVX;
+    a.WAVZ = c->WAVZ;
+    a.WL = c->WL;
+    a.WD = c->WD;
+    a.WR = c->WR;
+    a.Re = c->cfg.Re;
+    a.G_pre = c->cfg.G_pre;
+    a.g_v = c->g_v;
+    a.pa = c->pa;
+    a.pb = c->pb;
+    for (int k = 0; k < 3; ++k) {
+        a.aV[k] = c->cfg.a_V[k];
+        a.bV[k] = c->cfg.b_V[k];
+    }
+    a.ad_p = c->ad_p;
+    a.ad_v = c->ad_v;
+    for (int i = 0; i < 2; ++i) {
+        a.PI[i] = c->PI[i];
+        a.ID[i] = c->ID[i];
+        for (int k = 0; k < 3; ++k) a.VI[i][k] = c->VI[i][k];
+    }
+    a.PB = c->PB;
+    for (int k = 0; k < 3; ++k) a.VB[k] = c->VB[k];
+    a.SIG0 = c->SIG0;
+    a.u_unit = c->u_unit;
+    a.w_unit = c->w_unit;
+    a.q_unit = c->q_unit;
+    const double th = c->cfg.THETA * std::acos(-1.0) / 180.0;
+    a.grad_pre = c->cfg.INDEX_GRAD_PRE;
+    if (a.grad_pre == 1) {
+        a.grad_x = -std::cos(th);
+        a.grad_z = -std::sin(th);
+    } else {
+        a.qx_mean = std::cos(th);
+        a.qz_mean = std::sin(th);
+    }
+    a.vbc = c->vbc;
+    a.flow = c->flow;
+    return a;
+}
+
+// EqSolvePV::setup (eqsolve_pv.cpp:495-509), entirely on the device.
+void setup(cdns_ctx *c, double g_v) {
+    const Geometry &g = c->g;
+    c->g_v = g_v;
+    {
+        Launch L(c, "setup_ad");
+        launch_build_ad(c->WAVX, c->WAVZ, c->WL, c->WD, c->WR, c->cfg.G_pre, g.NYS, g.JS, g.NQ, g.nzk, c->ad_p,
+                        c->wn_p, c->st);
+        launch_build_ad(c->WAVX, c->WAVZ, c->WL, c->WD, c->WR, g_v, g.NYS, g.JS, g.NQ, g.nzk, c->ad_v, c->wn_v, c->st);
+    }
+    c->pa = 1.0;
+    c->pb = 0.0;
+    for (int i = 0; i < 2; ++i) {
+        SolveArgs a = solve_args(c, MODE_INFLUENCE);
+        a.influence_index = i;
+        a.P = c->PI[i];
+        for (int k = 0; k < 3; ++k) a.V[k] = c->VI[i][k];
+        a.div_out = c->ID[i];
+        Launch L(c, "setup_influence");
+        launch_solve(a, g.E, c->st);
+    }
+    {
+        Launch L(c, "setup_normalize");
+        launch_normalize_influence(c->ID[0], c->ID[1], g.NQ, g.NZS, c->cfg.a_V[1], c->st);
+    }
+    {
+        SolveArgs a = solve_args(c, MODE_BPROBLEM);
+        a.correct = 1;
+        a.P = c->PB;
+        for (int k = 0; k < 3; ++k) a.V[k] = c->VB[k];
+        a.sig_out = c->SIG0;
+        Launch L(c, "setup_bproblem");
+        launch_solve(a, g.E, c->st);
+    }
+    c->pa = c->cfg.a_pre0;
+    c->pb = c->cfg.b_pre0;
+    {
+        Launch L(c, "setup_pg_response");
+        launch_pg_response(c->WAVX, c->WAVZ, c->WL, c->WD, c->WR, c->ad_v, g_v, c->cfg.Re, c->cfg.a_V, c->cfg.b_V,
+                           g.NYS, g.JS, g.NZS, c->u_unit, c->w_unit, c->q_unit, g.E, c->st);
+    }
+    c->setup_done = true;
+}
+
+void step_coefficients(int it, double dt, double Re, double co[5]) {
+    if (it < 1 || dt <= 0.0) throw std::invalid_argument("invalid time-step coefficients");
+    const int eff = it < 3 ? it : 2;
+    const double a0 = (double)eff / dt, a1 = (double)(1 - eff) / (2.0 * dt);
+    co[0] = a0;
+    co[1] = a1;
+    co[2] = (double)eff;
+    co[3] = (double)(1 - eff);
+    co[4] = (a0 + a1) * Re;
+}
+
+// One iteration of Simulation::run (simulation.cpp:235-247) with enqueue_serial_step (204-233).
+void step(cdns_ctx *c, int it) {
+    double co[5];
+    step_coefficients(it, c->cfg.DT, c->cfg.Re, co);
+    if (it < 3) setup(c, co[4]);
+    if (!c->setup_done) throw std::logic_error("solver setup has not run (the first steps must be 1 and 2)");
+    c->time += c->cfg.DT;
+    nonlinear_planes(c, c->u, c->nl);
+    SolveArgs a = solve_args(c, MODE_STEP);
+    for (int k = 0; k < 3; ++k) {
+        a.u[k] = c->u[k];
+        a.up[k] = c->up[k];
+        a.nl[k] = c->nl[k];
+        a.nlp[k] = c->nlp[k];
+        a.V[k] = c->up[k]; // u^{n+1} replaces u^{n-1} in place
+    }
+    a.a0 = co[0];
+    a.a1 = co[1];
+    a.b0 = co[2];
+    a.b1 = co[3];
+    a.bcw = c->bcw;
+    a.bc_prev = c->bc_prev;
+    a.bc_ext = c->bc_ext;
+    a.P = c->P_;
+    a.correct = 1;
+    a.sigma = 1;
+    {
+        Launch L(c, "solve_step");
+        launch_solve(a, c->g.E, c->st);
+    }
+    for (int k = 0; k < 3; ++k) {
+        std::swap(c->u[k], c->up[k]);
+        std::swap(c->nl[k], c->nlp[k]);
+    }
+}
+
+// ---------------------------------------------------- layout conversion --
+// Reference-layout spectral array (host or device) -> internal planes.
+void ref_to_planes(cdns_ctx *c, const double *src, int stride, int ncomp, int nrow, zd *const *planes, int where,
+                   int JS) {
+    const Geometry &g = c->g;
+    const size_t n = (size_t)g.nzk * nrow * g.nkx;
+    const zd *dev = reinterpret_cast<const zd *>(src);
+    int dstride = stride;
+    if (where == CDNS_HOST) {
+        // gather the (possibly strided) components into the staging buffer
+        std::vector<zd> tmp(n * ncomp);
+        const zd *h = reinterpret_cast<const zd *>(src);
+        for (size_t e = 0; e < n; ++e)
+            for (int k = 0; k < ncomp; ++k) tmp[e * ncomp + k] = h[e * stride + k];
+        zd *stage = nrow == 2 ? c->bnd_stage : c->ref_stage;
+        CDNS_HIP(hipMemcpyAsync(stage, tmp.data(), tmp.size() * sizeof(zd), hipMemcpyHostToDevice, c->st));
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        dev = stage;
+        dstride = ncomp;
+    }
+    LayoutArgs a{};
+    a.ref_in = dev;
+    a.stride = dstride;
+    a.ncomp = ncomp;
+    a.nrow = nrow;
+    for (int k = 0; k < ncomp; ++k) a.plane_out[k] = planes[k];
+    a.nkx = g.nkx;
+    a.nzk = g.nzk;
+    a.JS = JS;
+    Launch L(c, "layout_in");
+    launch_ref_to_planes(a, c->st);
+}
+
+void planes_to_ref(cdns_ctx *c, const zd *const *planes, int ncomp, int nrow, double *dst, int stride, int where,
+                   int JS) {
+    const Geometry &g = c->g;
+    const size_t n = (size_t)g.nzk * nrow * g.nkx;
+    LayoutArgs a{};
+    a.nrow = nrow;
+    a.ncomp = ncomp;
+    for (int k = 0; k < ncomp; ++k) a.plane_in[k] = planes[k];
+    a.nkx = g.nkx;
+    a.nzk = g.nzk;
+    a.JS = JS;
+    if (where == CDNS_DEVICE) {
+        a.ref_out = reinterpret_cast<zd *>(dst);
+        a.stride = stride;
+        Launch L(c, "layout_out");
+        launch_planes_to_ref(a, c->st);
+        return;
+    }
+    zd *stage = nrow == 2 ? c->bnd_stage : c->ref_stage;
+    a.ref_out = stage;
+    a.stride = ncomp;
+    {
+        Launch L(c, "layout_out");
+        launch_planes_to_ref(a, c->st);
+    }
+    std::vector<zd> tmp(n * ncomp);
+    CDNS_HIP(hipMemcpyAsync(tmp.data(), stage, tmp.size() * sizeof(zd), hipMemcpyDeviceToHost, c->st));
+    CDNS_HIP(hipStreamSynchronize(c->st));
+    zd *h = reinterpret_cast<zd *>(dst);
+    for (size_t e = 0; e < n; ++e)
+        for (int k = 0; k < ncomp; ++k) h[e * stride + k] = tmp[e * ncomp + k];
+}
+
+int fail(cdns_ctx *c, const std::exception &e) {
+    if (c) c->err = e.what();
+    return -1;
+}
+
+} // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    auto c = std::make_unique<cdns_ctx>();
+    try {
+        c->cfg = *cfg;
+        Geometry &g = c->g;
+        g.NX = cfg->NX; g.NY = cfg->NY; g.NZ = cfg->NZ;
+        g.NXS = cfg->NXS; g.NYS = cfg->NYS; g.NZS = cfg->NZS;
+        g.NX2 = cfg->NX2; g.NY2 = cfg->NY2; g.NZ2 = cfg->NZ2;
+        if (g.NYS < 4 || g.NXS < 0 || g.NZS < 0 || 2 * g.NXS > g.NX2 || g.NY2 < g.NYS || 2 * g.NZS > g.NZ2 ||
+            g.NZ2 < 1 || g.NYS > 255)
+            throw std::invalid_argument("unsupported mesh sizes (need 2 NXS <= NX2, NYS <= NY2, 2 NZS <= NZ2, 4 <= NYS < 256)");
+        if (cfg->precision != CDNS_FP64 && cfg->precision != CDNS_MIXED)
+            throw std::invalid_argument("precision must be CDNS_FP64 or CDNS_MIXED");
+        c->fp32 = cfg->precision == CDNS_MIXED;
+        c->esz = c->fp32 ? 8 : 16;
+        g.nkx = g.NXS + 1;
+        g.nzk = 2 * g.NZS + 1;
+        g.nj = g.NYS + 1;
+        g.E = g.nj <= 128 ? 2 : 4;
+        g.JS = (g.nj + 7) / 8 * 8;
+        g.NQ = g.nkx * g.nzk;
+        g.NL = g.NY2 + 1;
+        g.NLH = g.NY2 / 2 + 1;
+        g.S = (size_t)g.NQ * g.JS;
+        g.YS = (size_t)g.NL * g.NQ;
+        g.ZS = (size_t)g.NL * g.NZ2 * g.nkx;
+        CDNS_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        cdns_ctx *C = c.get();
+
+        // wavenumbers and Chebyshev weights (transforms.cpp:71-108)
+        const double PI = std::acos(-1.0);
+        const double ALPHAX = 2.0 * PI / cfg->XL, ALPHAZ = 2.0 * PI / cfg->ZL;
+        C->hWAVX.resize(g.nkx);
+        C->hWAVZ.resize(g.nzk);
+        for (int k = 0; k < g.nkx; ++k) C->hWAVX[k] = (0 + k) * ALPHAX;
+        for (int k = -g.NZS; k <= g.NZS; ++k) C->hWAVZ[k + g.NZS] = k * ALPHAZ;
+        std::vector<double> WL(g.nj, 0.0), WD(g.nj, 0.0), WR(g.nj, 0.0);
+        for (int iy = g.NYS; iy >= 2; --iy) {
+            WL[iy] = 1.0 / (4.0 * iy * (iy - 1));
+            WD[iy] = -1.0 / (2.0 * (iy * iy - 1));
+            WR[iy] = 1.0 / (4.0 * iy * (iy + 1));
+        }
+        WL[2] = 2.0 * WL[2];
+        for (int iy = g.NYS - 1; iy <= g.NYS; ++iy) WD[iy] = 0.0;
+        for (int iy = g.NYS - 3; iy <= g.NYS; ++iy) WR[iy] = 0.0;
+        auto upload = [&](const std::vector<double> &h) {
+            double *d = C->alloc<double>(h.size());
+            CDNS_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, C->st));
+            return d;
+        };
+        C->WAVX = upload(C->hWAVX);
+        C->WAVZ = upload(C->hWAVZ);
+        C->WL = upload(WL);
+        C->WD = upload(WD);
+        C->WR = upload(WR);
+
+        // y-stage matrices from exact-angle cosines
+        const int N = g.NY2;
+        const auto cs = cospi_table(N);
+        std::vector<double> B((size_t)g.nj * g.NLH);
+        for (int j = 0; j < g.nj; ++j)
+            for (int l = 0; l < g.NLH; ++l) {
+                const long double bw = (j == N) ? 0.5L : 1.0L; // REDFT00 endpoint weight x the 0.5 z factor
+                B[(size_t)j * g.NLH + l] = (double)(bw * cs[((long long)j * l) % (2 * N)]);
+            }
+        const int nje = (g.nj + 1) / 2, njo = g.nj / 2;
+        std::vector<double> Fe((size_t)g.NLH * nje), Fo((size_t)g.NLH * std::max(njo, 1));
+        for (int l = 0; l < g.NLH; ++l) {
+            const long double wl = (l == 0) ? 1.0L : 2.0L;
+            for (int j = 0; j < g.nj; ++j) {
+                const long double s = j == 0 ? 1.0L / (2.0L * N) : 1.0L / N;
+                const double v = (double)(s * wl * cs[((long long)j * l) % (2 * N)]);
+                if (j % 2 == 0) Fe[(size_t)l * nje + j / 2] = v;
+                else Fo[(size_t)l * njo + j / 2] = v;
+            }
+        }
+        C->ycoef_b = upload(B);
+        C->ycoef_fe = upload(Fe);
+        C->ycoef_fo = upload(Fo);
+
+        // FFT plans and twiddles
+        C->pz = make_plan(g.NZ2);
+        C->px = make_plan(g.NX2);
+        auto up_tw = [&](int n) -> void * {
+            const auto t = twiddles(n);
+            if (C->fp32) {
+                std::vector<float> f(t.size());
+                for (size_t i = 0; i < t.size(); ++i) f[i] = (float)t[i];
+                float *d = C->alloc<float>(f.size());
+                CDNS_HIP(hipMemcpyAsync(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, C->st));
+                CDNS_HIP(hipStreamSynchronize(C->st));
+                return d;
+            }
+            std::vector<double> dv(t.size());
+            for (size_t i = 0; i < t.size(); ++i) dv[i] = (double)t[i];
+            double *d = upload(dv);
+            CDNS_HIP(hipStreamSynchronize(C->st));
+            return d;
+        };
+        C->twz = up_tw(g.NZ2);
+        C->twx = up_tw(g.NX2);
+        // z lines per workgroup: at most 4 butterflies per thread per stage
+        auto ipt = [](int lines, const FftPlan &p) { return (lines * (p.n / plan_min_radix(p)) + 255) / 256; };
+        int T = (g.nkx + 15) / 16;
+        C->KT = (g.nkx + T - 1) / T;
+        while (ipt(C->KT, C->pz) > 4 && C->KT > 1) {
+            ++T;
+            C->KT = (g.nkx + T - 1) / T;
+        }
+        // x-product points per workgroup (even), LDS <= 64 KB and <= 4 butterflies per thread
+        C->P = 2;
+        for (int P = 8; P >= 2; P -= 2) {
+            if (ipt(3 * P, C->px) <= 4 && (size_t)3 * P * g.NX2 * C->esz <= 65536 && (P / 2) * g.NX2 <= 8 * 256) {
+                C->P = P;
+                break;
+            }
+        }
+        if (ipt(3 * C->P, C->px) > 4 || ipt(C->KT, C->pz) > 4)
+            throw std::invalid_argument("product grid too large for one workgroup per line batch");
+
+        // solver tables
+        C->ad_p = C->alloc<double>(g.S);
+        C->ad_v = C->alloc<double>(g.S);
+        C->wn_p = C->alloc<double>(g.NQ);
+        C->wn_v = C->alloc<double>(g.NQ);
+        for (int i = 0; i < 2; ++i) {
+            C->PI[i] = C->alloc<zd>(g.S);
+            for (int k = 0; k < 3; ++k) C->VI[i][k] = C->alloc<zd>(g.S);
+            C->ID[i] = C->alloc<zd>((size_t)g.NQ * 2);
+        }
+        C->PB = C->alloc<zd>(g.S);
+        for (int k = 0; k < 3; ++k) C->VB[k] = C->alloc<zd>(g.S);
+        C->SIG0 = C->alloc<zd>((size_t)g.NQ * 2);
+        C->u_unit = C->alloc<double>(g.JS);
+        C->w_unit = C->alloc<double>(g.JS);
+        C->q_unit = C->alloc<double>(2);
+        C->pa = cfg->a_pre0;
+        C->pb = cfg->b_pre0;
+
+        // state
+        for (int k = 0; k < 3; ++k) {
+            C->u[k] = C->alloc<zd>(g.S);
+            C->up[k] = C->alloc<zd>(g.S);
+            C->nl[k] = C->alloc<zd>(g.S);
+            C->nlp[k] = C->alloc<zd>(g.S);
+            C->om[k] = C->alloc<zd>(g.S);
+        }
+        C->P_ = C->alloc<zd>(g.S);
+        C->bc_prev = C->alloc<zd>((size_t)g.NQ * 2);
+        C->bc_ext = C->alloc<zd>((size_t)g.NQ * 2);
+        C->bcw = C->alloc<zd>((size_t)g.NQ * 2);
+        C->vbc = C->alloc<zd>((size_t)g.NQ * 2 * 3);
+        C->flow = C->alloc<double>(2);
+        // velocity_boundary (initial_bc.cpp:95-101): wall-normal velocity on the mean mode
+        {
+            std::vector<zd> h((size_t)g.NQ * 2 * 3, zd{0.0, 0.0});
+            const size_t root = (size_t)g.NZS; // q of (kx=0, kz=0)
+            h[(size_t)1 * g.NQ * 2 + root * 2 + 0] = {cfg->V_Microblow0, 0.0};
+            h[(size_t)1 * g.NQ * 2 + root * 2 + 1] = {cfg->V_Microblow0, 0.0};
+            CDNS_HIP(hipMemcpyAsync(C->vbc, h.data(), h.size() * sizeof(zd), hipMemcpyHostToDevice, C->st));
+            CDNS_HIP(hipStreamSynchronize(C->st));
+        }
+
+        // workspaces
+        C->Ybuf = C->alloc<unsigned char>(6 * g.YS * C->esz);
+        C->Zbuf = C->alloc<unsigned char>(6 * g.ZS * C->esz);
+        C->ZNbuf = C->alloc<unsigned char>(3 * g.ZS * C->esz);
+        for (int i = 0; i < 8; ++i) C->scr[i] = C->alloc<zd>(g.S);
+        C->ref_stage = C->alloc<zd>((size_t)3 * g.nzk * g.nj * g.nkx);
+        C->bnd_stage = C->alloc<zd>((size_t)3 * g.nzk * 2 * g.nkx);
+        C->phys_stage = C->alloc<double>((size_t)g.NZ2 * g.NL * g.NX2);
+        CDNS_HIP(hipStreamSynchronize(C->st));
+    } catch (const std::exception &e) {
+        static std::string last;
+        last = e.what();
+        fprintf(stderr, "cdns_create: %s\n", e.what());
+        cdns_destroy(c.release());
+        return -1;
+    }
+    *out = c.release();
+    return 0;
+}
+
+void cdns_destroy(cdns_ctx *c) {
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto &e : c->pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char *cdns_last_error(const cdns_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void *cdns_stream(cdns_ctx *c) { return c ? (void *)c->st : nullptr; }
+
+int cdns_sync(cdns_ctx *c) {
+    try {
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        if (c->prof) drain_profile(c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_xyzfft2(cdns_ctx *c, char dir, double *phys, double *spec, int stride, int where) {
+    try {
+        if (stride < 1) throw std::invalid_argument("spectral stride must be positive");
+        if (dir != 'F' && dir != 'B') throw std::invalid_argument("direction must be 'F' or 'B'");
+        const Geometry &g = c->g;
+        const size_t np = (size_t)g.NZ2 * g.NL * g.NX2;
+        double *dphys = where == CDNS_DEVICE ? phys : c->phys_stage;
+        if (dir == 'B') {
+            zd *pl[1] = {c->scr[0]};
+            ref_to_planes(c, spec, stride, 1, g.nj, pl, where, g.JS);
+            const zd *cp[1] = {c->scr[0]};
+            backward_yz(c, cp, 1);
+            XArgs a = xargs(c, c->Zbuf, nullptr, g.ZS, g.ZS, 2 * c->P);
+            a.phys = dphys;
+            {
+                Launch L(c, "x_backward");
+                launch_xbwd_phys(a, c->fp32, c->st);
+            }
+            if (where == CDNS_HOST) {
+                CDNS_HIP(hipMemcpyAsync(phys, dphys, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+                CDNS_HIP(hipStreamSynchronize(c->st));
+            }
+        } else {
+            if (where == CDNS_HOST)
+                CDNS_HIP(hipMemcpyAsync(dphys, phys, np * sizeof(double), hipMemcpyHostToDevice, c->st));
+            XArgs a = xargs(c, nullptr, c->ZNbuf, g.ZS, g.ZS, 2 * c->P);
+            a.phys = dphys;
+            {
+                Launch L(c, "x_forward");
+                launch_xfwd_phys(a, c->fp32, c->st);
+            }
+            zd *pl[1] = {c->scr[0]};
+            forward_zy(c, pl, 1);
+            const zd *cp[1] = {c->scr[0]};
+            planes_to_ref(c, cp, 1, g.nj, spec, stride, where, g.JS);
+        }
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_curl(cdns_ctx *c, double *out, const double *in, int where) {
+    try {
+        const Geometry &g = c->g;
+        zd *a[3] = {c->scr[0], c->scr[1], c->scr[2]}, *b[3] = {c->scr[3], c->scr[4], c->scr[5]};
+        ref_to_planes(c, in, 3, 3, g.nj, a, where, g.JS);
+        curl_planes(c, a, b, nullptr);
+        const zd *cb[3] = {b[0], b[1], b[2]};
+        planes_to_ref(c, cb, 3, g.nj, out, 3, where, g.JS);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_cross_product(cdns_ctx *c, double *out, const double *av, const double *bv, int where) {
+    try {
+        const Geometry &g = c->g;
+        zd *a[3] = {c->scr[0], c->scr[1], c->scr[2]}, *b[3] = {c->scr[3], c->scr[4], c->scr[5]};
+        ref_to_planes(c, av, 3, 3, g.nj, a, where, g.JS);
+        ref_to_planes(c, bv, 3, 3, g.nj, b, where, g.JS);
+        zd *o[3] = {c->scr[0], c->scr[1], c->scr[2]};
+        cross_planes(c, a, b, o);
+        const zd *co[3] = {o[0], o[1], o[2]};
+        planes_to_ref(c, co, 3, g.nj, out, 3, where, g.JS);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_nonlinear_term(cdns_ctx *c, double *nl, const double *velocity, double *pressure_bc, int where) {
+    try {
+        const Geometry &g = c->g;
+        zd *u[3] = {c->scr[0], c->scr[1], c->scr[2]}, *n[3] = {c->scr[3], c->scr[4], c->scr[5]};
+        ref_to_planes(c, velocity, 3, 3, g.nj, u, where, g.JS);
+        nonlinear_planes(c, u, n);
+        {
+            Launch L(c, "wall_sum");
+            launch_wall_sum(n[1], c->bcw, c->scr[6], g.NYS, g.JS, g.NQ, g.E, c->st);
+        }
+        const zd *cn[3] = {n[0], n[1], n[2]};
+        planes_to_ref(c, cn, 3, g.nj, nl, 3, where, g.JS);
+        const zd *pb[1] = {c->scr[6]};
+        planes_to_ref(c, pb, 1, 2, pressure_bc, 1, where, 2);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_setup(cdns_ctx *c, double g_v) {
+    try {
+        setup(c, g_v);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_solve_modes(cdns_ctx *c, double *pressure, const double *rhs_p, const double *pressure_bc, double *velocity,
+                     const double *rhs_v, const double *velocity_bc, int correct_sigma, int where) {
+    try {
+        if (!c->setup_done) throw std::logic_error("cdns_setup must run before cdns_solve_modes");
+        const Geometry &g = c->g;
+        zd *rv[3] = {c->scr[0], c->scr[1], c->scr[2]};
+        ref_to_planes(c, rhs_v, 3, 3, g.nj, rv, where, g.JS);
+        zd *rp = nullptr;
+        if (rhs_p) {
+            zd *pl[1] = {c->scr[3]};
+            ref_to_planes(c, rhs_p, 1, 1, g.nj, pl, where, g.JS);
+            rp = c->scr[3];
+        }
+        // boundaries: pressure [q][2] in scr[6]; velocity [c][q][2] in a temporary set of planes
+        zd *pbc[1] = {c->scr[6]};
+        ref_to_planes(c, pressure_bc, 1, 1, 2, pbc, where, 2);
+        zd *vb = c->scr[7];
+        zd *vbp[3] = {vb, vb + (size_t)g.NQ * 2, vb + (size_t)g.NQ * 4};
+        ref_to_planes(c, velocity_bc, 3, 3, 2, vbp, where, 2);
+        SolveArgs a = solve_args(c, MODE_GIVEN);
+        for (int k = 0; k < 3; ++k) a.rv[k] = rv[k];
+        a.rp = rp;
+        a.pbc = c->scr[6];
+        a.vbc = vb;
+        a.correct = 1;
+        a.sigma = correct_sigma ? 1 : 0;
+        a.P = c->scr[4];
+        zd *V[3] = {c->scr[0], c->scr[1], c->scr[5]};
+        for (int k = 0; k < 3; ++k) a.V[k] = V[k];
+        // V[0], V[1] alias the rhs planes of the same column: each wavefront reads its
+        // column before it writes it.
+        {
+            Launch L(c, "solve_modes");
+            launch_solve(a, g.E, c->st);
+        }
+        const zd *cp[1] = {c->scr[4]};
+        planes_to_ref(c, cp, 1, g.nj, pressure, 1, where, g.JS);
+        const zd *cv[3] = {V[0], V[1], V[2]};
+        planes_to_ref(c, cv, 3, g.nj, velocity, 3, where, g.JS);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_get_table(cdns_ctx *c, int which, double *out) {
+    try {
+        const Geometry &g = c->g;
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        auto real_plane = [&](const double *plane, int nrow, int JS) {
+            double *d = reinterpret_cast<double *>(c->ref_stage);
+            launch_real_planes_to_ref(plane, d, nrow, g.nkx, g.nzk, JS, c->st);
+            CDNS_HIP(hipMemcpyAsync(out, d, (size_t)g.nzk * nrow * g.nkx * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        };
+        switch (which) {
+        case 0: real_plane(c->wn_p, 1, 1); break;
+        case 1: real_plane(c->wn_v, 1, 1); break;
+        case 2: real_plane(c->ad_p, g.nj, g.JS); break;
+        case 3: real_plane(c->ad_v, g.nj, g.JS); break;
+        case 4: case 5: { const zd *p[1] = {c->PI[which - 4]}; planes_to_ref(c, p, 1, g.nj, out, 1, CDNS_HOST, g.JS); break; }
+        case 6: case 7: { const zd *p[3] = {c->VI[which - 6][0], c->VI[which - 6][1], c->VI[which - 6][2]}; planes_to_ref(c, p, 3, g.nj, out, 3, CDNS_HOST, g.JS); break; }
+        case 8: case 9: { const zd *p[1] = {c->ID[which - 8]}; planes_to_ref(c, p, 1, 2, out, 1, CDNS_HOST, 2); break; }
+        case 10: { const zd *p[1] = {c->PB}; planes_to_ref(c, p, 1, g.nj, out, 1, CDNS_HOST, g.JS); break; }
+        case 11: { const zd *p[3] = {c->VB[0], c->VB[1], c->VB[2]}; planes_to_ref(c, p, 3, g.nj, out, 3, CDNS_HOST, g.JS); break; }
+        case 12: { const zd *p[1] = {c->SIG0}; planes_to_ref(c, p, 1, 2, out, 1, CDNS_HOST, 2); break; }
+        case 13: CDNS_HIP(hipMemcpyAsync(out, c->u_unit, g.nj * sizeof(double), hipMemcpyDeviceToHost, c->st)); break;
+        case 14: CDNS_HIP(hipMemcpyAsync(out, c->w_unit, g.nj * sizeof(double), hipMemcpyDeviceToHost, c->st)); break;
+        case 15: CDNS_HIP(hipMemcpyAsync(out, c->q_unit, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st)); break;
+        default: throw std::invalid_argument("unknown table");
+        }
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+static void state_planes(cdns_ctx *c, int field, zd **pl, int &ncomp, int &nrow, int &JS) {
+    const Geometry &g = c->g;
+    nrow = g.nj;
+    JS = g.JS;
+    ncomp = 3;
+    switch (field) {
+    case 0: for (int k = 0; k < 3; ++k) pl[k] = c->u[k]; break;
+    case 1: for (int k = 0; k < 3; ++k) pl[k] = c->up[k]; break;
+    case 2: for (int k = 0; k < 3; ++k) pl[k] = c->nl[k]; break;
+    case 3: for (int k = 0; k < 3; ++k) pl[k] = c->nlp[k]; break;
+    case 4: ncomp = 1; pl[0] = c->P_; break;
+    case 5: ncomp = 1; nrow = 2; JS = 2; pl[0] = c->bc_ext; break;
+    case 6: ncomp = 1; nrow = 2; JS = 2; pl[0] = c->bc_prev; break;
+    case 7: nrow = 2; JS = 2; for (int k = 0; k < 3; ++k) pl[k] = c->vbc + (size_t)k * g.NQ * 2; break;
+    default: throw std::invalid_argument("unknown state field");
+    }
+}
+
+int cdns_set_state(cdns_ctx *c, int field, const double *data, int where) {
+    try {
+        zd *pl[3] = {};
+        int ncomp, nrow, JS;
+        state_planes(c, field, pl, ncomp, nrow, JS);
+        ref_to_planes(c, data, ncomp, ncomp, nrow, pl, where, JS);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_get_state(cdns_ctx *c, int field, double *data, int where) {
+    try {
+        zd *pl[3] = {};
+        int ncomp, nrow, JS;
+        state_planes(c, field, pl, ncomp, nrow, JS);
+        const zd *cp[3] = {pl[0], pl[1], pl[2]};
+        planes_to_ref(c, cp, ncomp, nrow, data, ncomp, where, JS);
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_step(cdns_ctx *c, int it) {
+    try {
+        step(c, it);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_run(cdns_ctx *c, int it_begin, int nsteps) {
+    try {
+        for (int i = 0; i < nsteps; ++i) step(c, it_begin + i);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_flow_gradients(cdns_ctx *c, double out[2]) {
+    try {
+        CDNS_HIP(hipMemcpyAsync(out, c->flow, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+double cdns_time(const cdns_ctx *c) { return c ? c->time : 0.0; }
+
+int cdns_profile(cdns_ctx *c, int enable) {
+    try {
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        drain_profile(c);
+        c->prof = enable != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_profile_read(cdns_ctx *c, char *names, long long *counts, double *total_ms, int cap) {
+    try {
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        drain_profile(c);
+        const int n = (int)c->kname.size();
+        for (int i = 0; i < n && i < cap; ++i) {
+            if (names) {
+                std::memset(names + 64 * i, 0, 64);
+                std::strncpy(names + 64 * i, c->kname[i].c_str(), 63);
+            }
+            if (counts) counts[i] = c->kcnt[i];
+            if (total_ms) total_ms[i] = c->ktot[i];
+        }
+        return n;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_profile_reset(cdns_ctx *c) {
+    try {
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        drain_profile(c);
+        std::fill(c->ktot.begin(), c->ktot.end(), 0.0);
+        std::fill(c->kcnt.begin(), c->kcnt.end(), 0LL);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,117 @@
+// Kernel argument blocks and launcher declarations shared by the backend's sources.
+#pragma once
+
+#include "common.hpp"
+
+namespace cdns {
+
+// y stages (dense Chebyshev apply).  coef: backward [j][NLH]; forward even/odd [NLH][nje|njo].
+struct YArgs {
+    const zd *spec[6]; // ybwd inputs: spectral component planes
+    zd *spec_out[6];   // yfwd outputs
+    void *y;           // Y fields, field f at y + f*YS (element type by precision)
+    const double *coef, *coef2;
+    int nj, nje, njo, JS, NQ, NLH, NY2, nwj, ng;
+    size_t YS;
+};
+
+// z stages (FFT over kz).
+struct ZArgs {
+    const void *in;
+    void *out;
+    size_t in_fs, out_fs; // field strides, complex elements
+    const void *tw;
+    FftPlan plan;
+    int nkx, nzk, NZS, NQ, KT, NL;
+};
+
+// x stages (FFT over kx) and the cross product.
+struct XArgs {
+    const void *in;
+    void *out;
+    size_t in_fs, out_fs;
+    const void *tw;
+    FftPlan plan;
+    int nkx, NL, NZ2, P;
+    double *phys; // reference-layout physical field for the single-field stages
+};
+
+void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
+void launch_yfwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
+void launch_zbwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);
+void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);
+void launch_xprod(const XArgs &a, bool fp32, hipStream_t st);
+void launch_xbwd_phys(const XArgs &a, bool fp32, hipStream_t st);
+void launch_xfwd_phys(const XArgs &a, bool fp32, hipStream_t st);
+
+// ----------------------------------------------------------- spectral ----
+struct LayoutArgs {
+    // reference layout element (izi, j, kx, c) at ref[((izi*nrow + j)*nkx + kx)*stride + c]
+    const zd *ref_in;
+    zd *ref_out;
+    int stride, ncomp, nrow; // nrow: NYS+1 for fields, 2 for boundaries
+    const zd *plane_in[3];
+    zd *plane_out[3];
+    int nkx, nzk, JS; // internal column stride (JS for fields, 2 for boundaries)
+};
+void launch_ref_to_planes(const LayoutArgs &a, hipStream_t st);
+void launch_planes_to_ref(const LayoutArgs &a, hipStream_t st);
+void launch_real_planes_to_ref(const double *plane, double *ref, int nrow, int nkx, int nzk, int JS, hipStream_t st);
+
+struct CurlArgs {
+    const zd *u[3];
+    zd *w[3];
+    zd *bcw; // [q][2] curl part of the pressure wall condition, or null
+    const double *WAVX, *WAVZ;
+    double Re;
+    int ny, JS, NQ, nzk;
+};
+void launch_curl(const CurlArgs &a, int E, hipStream_t st);
+// pressure wall condition = curl part + sum_j (+-1)^j N_y,j (timestepper.cpp:55-72)
+void launch_wall_sum(const zd *nly, const zd *bcw, zd *pbc, int ny, int JS, int NQ, int E, hipStream_t st);
+
+// ------------------------------------------------------------- solver ----
+enum SolveMode { MODE_STEP = 0, MODE_GIVEN = 1, MODE_INFLUENCE = 2, MODE_BPROBLEM = 3 };
+
+struct SolveArgs {
+    int mode;
+    int ny, JS, NQ, nzk, nkx;
+    const double *WAVX, *WAVZ, *WL, *WD, *WR;
+    double Re, G_pre, g_v, pa, pb; // pressure BC a/b currently in force
+    double aV[3], bV[3];
+    // rhs sources
+    const zd *u[3], *up[3], *nl[3], *nlp[3]; // step: a0 u + a1 up + b0 nl + b1 nlp
+    double a0, a1, b0, b1;
+    const zd *rv[3]; // given velocity rhs planes
+    const zd *rp;    // given pressure rhs plane (null: fused divergence)
+    const zd *pbc;   // given pressure BC [q][2]
+    const zd *vbc;   // velocity BC [c][q][2] (3 planes of NQ*2)
+    const zd *bcw;   // step: curl part of the pressure wall condition [q][2]
+    zd *bc_prev;     // step: in B^{n-1}, out B^n [q][2]
+    zd *bc_ext;      // step: out b0 B^n + b1 B^{n-1} [q][2]
+    int influence_index;
+    int correct, sigma;
+    // tables
+    const double *ad_p, *ad_v;
+    const zd *PI[2], *VI[2][3], *ID[2], *PB, *VB[3], *SIG0;
+    const double *u_unit, *w_unit;
+    const double *q_unit;
+    double grad_x, grad_z, qx_mean, qz_mean;
+    int grad_pre;
+    // outputs
+    zd *P, *V[3];
+    zd *div_out;  // influence: raw wall divergence [q][2]
+    zd *sig_out;  // b problem: sigma0 [q][2]
+    double *flow; // step: dpdx, dpdz
+};
+void launch_solve(const SolveArgs &a, int E, hipStream_t st);
+void launch_build_ad(const double *WAVX, const double *WAVZ, const double *WL, const double *WD,
+                     const double *WR, double g, int ny, int JS, int NQ, int nzk, double *ad,
+                     double *wn, hipStream_t st);
+void launch_normalize_influence(zd *ID0, zd *ID1, int NQ, int root_q, double aV1, hipStream_t st);
+void launch_pg_response(const double *WAVX, const double *WAVZ, const double *WL, const double *WD,
+                        const double *WR, const double *ad_v, double g_v, double Re, const double *aV,
+                        const double *bV, int ny, int JS, int root_q, double *u_unit, double *w_unit,
+                        double *q_unit, int E, hipStream_t st);
+
+} // namespace cdns

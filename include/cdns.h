@@ -1,0 +1,107 @@
+/* cdns.h -- C ABI of the MI355X (gfx950) backend for the channel-dns per-step hot path.
+ *
+ * Plain pointers and sizes only.  A context owns one HIP stream and every device
+ * buffer of one serial (single-rank) channel-flow run; calls are enqueued on the
+ * stream in order (the FIFO-batch semantics of the reference's CpuDevice,
+ * src/core/cpu_device.hpp:30-130) and cdns_sync() waits for them.  Every entry
+ * point returns 0 on success and a nonzero code on failure; cdns_last_error()
+ * gives the message.  Errors detected by the device are reported at the next
+ * cdns_sync(), as CpuDevice::sync rethrows (src/core/cpu_device.cpp:169-177).
+ *
+ * Layouts are the reference's host layouts (src/solver/fields.hpp:17-48):
+ *   spectral scalar  [(iz+NZS)][iy][kx]       complex (re,im), iy<=NYS, kx<=NXS
+ *   spectral vector  [(iz+NZS)][iy][kx][c]    c fastest
+ *   boundary         [(iz+NZS)][wall][kx][c]
+ *   fine physical    [iz][iy][ix]             real, NZ2 x (NY2+1) x NX2
+ * `where` selects host (CDNS_HOST) or device (CDNS_DEVICE) pointers.  Device
+ * pointers must stay valid until the next cdns_sync().
+ */
+#ifndef CDNS_H
+#define CDNS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDNS_HOST 0
+#define CDNS_DEVICE 1
+
+#define CDNS_FP64 0  /* every stage in binary64 */
+#define CDNS_MIXED 1 /* transforms and products in binary32; state, solves, mean mode in binary64 */
+
+/* Params + Mesh of one serial run (src/core/params.hpp:6-51, src/core/mesh.hpp:15-26). */
+typedef struct cdns_config {
+    int NX, NY, NZ;    /* coarse grid */
+    int NXS, NYS, NZS; /* retained modes */
+    int NX2, NY2, NZ2; /* product grid */
+    double XL, ZL, THETA, Re, G_pre, a_pre0, b_pre0;
+    double a_V[3], b_V[3];
+    int INDEX_GRAD_PRE;
+    double DT, V_Microblow0;
+    int precision; /* CDNS_FP64 or CDNS_MIXED */
+} cdns_config;
+
+typedef struct cdns_ctx cdns_ctx;
+
+/* Replaces the construction of Transforms + EqSolvePV + TimeStepper + Simulation
+ * buffers (src/main.cpp:192-221; src/sim/simulation.cpp:109-129). */
+int cdns_create(const cdns_config *cfg, cdns_ctx **out);
+void cdns_destroy(cdns_ctx *ctx);
+const char *cdns_last_error(const cdns_ctx *ctx);
+/* CpuDevice::sync (src/core/cpu_device.cpp:169-177). */
+int cdns_sync(cdns_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own work with it. */
+void *cdns_stream(cdns_ctx *ctx);
+
+/* Transforms::channel_xyzfft2 / enqueue_xyzfft2 (src/fft/transforms.hpp:40-43,
+ * transforms.cpp:588-741): dir 'B' synthesises the product-grid field from the
+ * spectral component at spec[0], spec[stride], ...; 'F' writes only that component. */
+int cdns_xyzfft2(cdns_ctx *ctx, char dir, double *phys, double *spec, int stride, int where);
+
+/* vector_ops::curl and vector_ops::cross_product (src/solver/vector_ops.hpp:22,33). */
+int cdns_curl(cdns_ctx *ctx, double *out, const double *in, int where);
+int cdns_cross_product(cdns_ctx *ctx, double *out, const double *a, const double *b, int where);
+
+/* TimeStepper::nonlinear_term (src/solver/timestepper.hpp:10-11): N = u x curl(u) and
+ * the nonlinear pressure wall condition. */
+int cdns_nonlinear_term(cdns_ctx *ctx, double *nl, const double *velocity, double *pressure_bc,
+                        int where);
+
+/* EqSolvePV::setup (src/solver/eqsolve_pv.hpp:17). */
+int cdns_setup(cdns_ctx *ctx, double g_v);
+/* EqSolvePV::solve_modes (eqsolve_pv.hpp:48-51); rhs_p == NULL forms div(rhs_v) in place.
+ * solve_coupled (eqsolve_pv.hpp:18-20) is rhs_p != NULL with correct_sigma = 0. */
+int cdns_solve_modes(cdns_ctx *ctx, double *pressure, const double *rhs_p, const double *pressure_bc,
+                     double *velocity, const double *rhs_v, const double *velocity_bc,
+                     int correct_sigma, int where);
+/* Setup tables in reference layout; `which` numbers as the EqSolvePV members
+ * (eqsolve_pv.hpp:61-75): 0 wn_p 1 wn_v 2 ad_p 3 ad_v 4 P_infl0 5 P_infl1 6 V_infl0
+ * 7 V_infl1 8 infl_div0 9 infl_div1 10 P_b 11 V_b 12 sigma0 13 u_dpdx 14 w_dpdz 15 {qx,qz}. */
+int cdns_get_table(cdns_ctx *ctx, int which, double *out);
+
+/* Simulation state (src/sim/simulation.hpp:44-55).  field: 0 velocity, 1 velocity_prev,
+ * 2 nonlinear, 3 nonlinear_prev, 4 pressure, 5 pressure_bc, 6 pressure_bc_prev, 7 velocity_bc.
+ * The multistep history (1, 3, 6) is what a restart file does not hold. */
+int cdns_set_state(cdns_ctx *ctx, int field, const double *data, int where);
+int cdns_get_state(cdns_ctx *ctx, int field, double *data, int where);
+/* One iteration `it` (>= 1) of Simulation::run's serial body: step_coefficients, setup on
+ * it < 3, enqueue_serial_step and correct_flow (src/sim/simulation.cpp:204-247). */
+int cdns_step(cdns_ctx *ctx, int it);
+/* Iterations it_begin .. it_begin+nsteps-1. */
+int cdns_run(cdns_ctx *ctx, int it_begin, int nsteps);
+/* dpdx_, dpdz_ after the last step (simulation.cpp:173-202) and the simulated time. */
+int cdns_flow_gradients(cdns_ctx *ctx, double out[2]);
+double cdns_time(const cdns_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context stream (no reference counterpart;
+ * the reference times only the whole loop, src/main.cpp:223-234). */
+int cdns_profile(cdns_ctx *ctx, int enable);
+/* Fills up to `cap` entries: kernel name (NUL-separated in `names`, 64 bytes each),
+ * launch count and total milliseconds; returns the number of kernels. */
+int cdns_profile_read(cdns_ctx *ctx, char *names, long long *counts, double *total_ms, int cap);
+int cdns_profile_reset(cdns_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""Parity of the HIP backend against the CPU oracle, through the C ABI (FP64 path).
+
+Tolerances: transforms and single operators 1e-13 (max-abs relative), full steps
+1e-11 relative L2 after one step and 1e-9 after 20 steps on every velocity component.
+"""
+import numpy as np
+import pytest
+
+from cdns import config, fields
+from helpers import band_spectrum, maxrel, ref_transform_config, stage3_config
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = {
+    "ref32_full": ref_transform_config(16, 30, 16),
+    "ref32_trunc": ref_transform_config(10, 20, 10),
+    "channel_32": config.channel_32(),
+    "channel_64": config.channel_64(),
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_fine_transform_backward_forward(gpu, name):
+    cfg = CONFIGS[name]
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    spec = band_spectrum(cfg, 0xF5)
+    p_dev, p_ref = dev.fine_phys(), ref.fine_phys()
+    dev.xyzfft2("B", p_dev, spec.copy())
+    ref.xyzfft2("B", p_ref, spec.copy())
+    assert maxrel(p_dev, p_ref) < 1e-13
+    s_dev, s_ref = dev.scalar(), ref.scalar()
+    dev.xyzfft2("F", p_ref.copy(), s_dev)
+    ref.xyzfft2("F", p_ref.copy(), s_ref)
+    assert maxrel(s_dev, s_ref) < 1e-13
+    # round trip of a realisable band-limited spectrum (tests/test_xyzfft.cpp:358-365)
+    assert maxrel(s_dev, spec) < 1e-12
+
+
+def test_strided_component_writes_only_its_slot(gpu):
+    """Strided transform integrity (tests/test_xyzfft.cpp:375-407)."""
+    cfg = CONFIGS["channel_32"]
+    dev = gpu.Channel(cfg)
+    src = band_spectrum(cfg, 0xAFF1)
+    guard = -7.0 + 3.0j
+    vin = np.full(src.shape + (3,), guard, np.complex128)
+    vin[..., 1] = src
+    phys = dev.fine_phys()
+    dev.xyzfft2("B", phys, vin, stride=3, offset=1)
+    expect = dev.fine_phys()
+    dev.xyzfft2("B", expect, src.copy())
+    assert np.array_equal(phys, expect)
+    out = np.full(src.shape + (3,), guard, np.complex128)
+    dev.xyzfft2("F", phys, out, stride=3, offset=2)
+    assert (out[..., 0] == guard).all() and (out[..., 1] == guard).all()
+    assert maxrel(out[..., 2], src) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["channel_32", "channel_64"])
+def test_curl_cross_nonlinear(gpu, name):
+    cfg = CONFIGS[name]
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    # a generic realisable field (no wall conditions), so the pressure wall term is O(1)
+    u = np.ascontiguousarray(np.stack([band_spectrum(cfg, 30 + c) for c in range(3)], axis=-1))
+    assert maxrel(dev.curl(u), ref.curl(u)) < 1e-13
+    w = ref.curl(u)
+    assert maxrel(dev.cross_product(u, w), ref.cross_product(u, w)) < 1e-12
+    nl_d, bc_d = dev.nonlinear_term(u)
+    nl_r, bc_r = ref.nonlinear_term(u)
+    assert maxrel(nl_d, nl_r) < 1e-12
+    assert maxrel(bc_d, bc_r) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["channel_32", "stage3"])
+def test_setup_tables(gpu, name):
+    cfg = stage3_config() if name == "stage3" else CONFIGS[name]
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    g = 1.5 / cfg["DT"] * cfg["Re"]
+    dev.setup(g)
+    ref.setup(g)
+    for t in pyoracle.Oracle.TABLES:
+        a, b = dev.table(t), ref.table(t)
+        assert maxrel(a, b) < 1e-11, t
+
+
+@pytest.mark.parametrize("correct", [False, True])
+@pytest.mark.parametrize("fuse", [False, True])
+def test_solve_modes(gpu, correct, fuse):
+    cfg = stage3_config()
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    dev.setup(150.0)
+    ref.setup(150.0)
+    rp = ref.scalar()
+    i = np.arange(rp.size).reshape(rp.shape)
+    rp[...] = np.sin(i * .13) * .001 + 1j * np.cos(i * .19) * .002
+    rv = ref.vector()
+    i = np.arange(rv.size).reshape(rv.shape)
+    rv[...] = np.sin(i * .17) * .001 + 1j * np.cos(i * .23) * .002
+    bp = ref.boundary()
+    i = np.arange(bp.size).reshape(bp.shape)
+    bp[...] = np.sin(i * .11) * .001 + 1j * np.cos(i * .29) * .002
+    bv = ref.boundary(3)
+    i = np.arange(bv.size).reshape(bv.shape)
+    bv[...] = np.sin(i * .31) * .001 + 1j * np.cos(i * .37) * .002
+    P1, V1 = dev.solve_modes(None if fuse else rp, bp, rv, bv, correct)
+    P2, V2 = ref.solve_modes(None if fuse else rp, bp, rv, bv, correct)
+    assert maxrel(P1, P2) < 1e-10
+    assert maxrel(V1, V2) < 1e-10
+
+
+@pytest.mark.parametrize("name,steps,tol", [("channel_32", 1, 1e-11), ("channel_32", 5, 1e-10),
+                                            ("channel_32", 20, 1e-9), ("stage3", 5, 1e-10)])
+def test_step_parity(gpu, name, steps, tol):
+    cfg = stage3_config() if name == "stage3" else CONFIGS[name]
+    u0 = fields.seeded_velocity(cfg, seed=5)
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    dev.set_state("velocity", u0)
+    ref.set_state("velocity", u0)
+    dev.run(steps, it_begin=1)
+    ref.run(steps)
+    got, want = dev.get_state("velocity"), ref.get_state("velocity")
+    errs = fields.rel_l2_components(got, want)
+    assert max(errs) < tol, errs
+    for f in ("velocity_prev", "nonlinear_prev", "pressure_bc_prev"):
+        assert fields.rel_l2(dev.get_state(f), ref.get_state(f)) < 10 * tol, f
+    assert fields.rel_l2(dev.get_state("pressure"), ref.get_state("pressure")) < 10 * tol
+    # constant-flux dpdx = (Q_target - Q(u*)) / Q(u_G) cancels, so its error is measured against
+    # the scale 1/Q(u_G) of one unit of flow-rate error (simulation.cpp:187-188)
+    fd, fr = dev.flow_gradients(), ref.flow_gradients()
+    scale = 1.0 / abs(ref.table("q_unit")[0]) if cfg["INDEX_GRAD_PRE"] != 1 else 1.0
+    assert np.all(np.abs(fd - fr) <= 1e-12 * scale), (fd, fr, scale)
+
+
+_RT180 = {}
+
+
+def _retau180_reference(steps):
+    """Oracle state after `steps` steps of the bench workload (cached per process)."""
+    import os
+
+    cfg = config.retau180()
+    if "u0" not in _RT180:
+        _RT180["u0"] = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
+        pyoracle.set_threads(min(16, os.cpu_count() or 1))
+        _RT180["oracle"] = pyoracle.Oracle(cfg)
+        _RT180["oracle"].set_state("velocity", _RT180["u0"])
+        _RT180["done"] = 0
+    o = _RT180["oracle"]
+    while _RT180["done"] < steps:
+        _RT180["done"] += 1
+        o.step(_RT180["done"])
+        _RT180[_RT180["done"]] = o.get_state("velocity")
+    return cfg, _RT180["u0"], _RT180[steps]
+
+
+@pytest.mark.parametrize("steps,tol", [(1, 1e-11), (20, 1e-9)])
+def test_step_parity_retau180(gpu, steps, tol):
+    """The bench workload (128x129x128, product 196x193x196): FP64 step vs the oracle."""
+    cfg, u0, want = _retau180_reference(steps)
+    dev = gpu.Channel(cfg)
+    dev.set_state("velocity", u0)
+    dev.run(steps, it_begin=1)
+    got = dev.get_state("velocity")
+    errs = fields.rel_l2_components(got, want)
+    print(f"retau180 {steps} steps rel-L2 per component: {errs}")
+    assert max(errs) < tol, errs
