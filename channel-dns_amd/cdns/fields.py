@@ -99,14 +99,15 @@ def mean_profile(cfg, kind):
     return c, float(up.max())
 
 
-def seeded_velocity(cfg, seed=1234, mean="laminar", spectrum="band", kmax=4, amplitude=0.1):
+def seeded_velocity(cfg, seed=1234, mean="laminar", spectrum="band", kmax=4, amplitude=0.1, kcut=None):
     """Initial spectral velocity [(iz+NZS)][iy][kx][c] = mean profile + curl(A) perturbation.
 
     spectrum='band': phi has random modes with |kx|,|kz|,j-index <= kmax; the
         perturbation is scaled so that max |u'| on the coarse grid = amplitude*centreline.
     spectrum='k53':  phi has random phases and |u'_k| ~ |k|^(-11/6) (a k^-5/3 energy
-        spectrum) over the retained modes below 2/3 of the cut-off; scaled to an rms of
-        amplitude*centreline on the coarse grid.
+        spectrum) over the retained modes below 2/3 of the cut-off, or with kx, |kz| and the
+        Chebyshev index all <= kcut when kcut is given; scaled to an rms of amplitude*centreline
+        on the coarse grid.
     """
     NXS, NYS, NZS = cfg["NXS"], cfg["NYS"], cfg["NZS"]
     nzk, nj, nkx = 2 * NZS + 1, NYS + 1, NXS + 1
@@ -130,8 +131,11 @@ def seeded_velocity(cfg, seed=1234, mean="laminar", spectrum="band", kmax=4, amp
             KY = (np.pi / 2.0) * np.arange(nj)[None, :, None]
             kk = np.sqrt(KX ** 2 + KZ ** 2 + KY ** 2)
             kk[kk == 0] = 1.0
-            mask = (np.abs(np.arange(-NZS, NZS + 1))[:, None, None] <= (2 * NZS) // 3) & \
-                   (np.arange(nkx)[None, None, :] <= (2 * NXS) // 3) & (np.arange(nj)[None, :, None] <= min(jmax, (2 * NYS) // 3))
+            cz, cx, cy = (2 * NZS) // 3, (2 * NXS) // 3, min(jmax, (2 * NYS) // 3)
+            if kcut is not None:
+                cz, cx, cy = min(cz, kcut), min(cx, kcut), min(cy, kcut)
+            mask = (np.abs(np.arange(-NZS, NZS + 1))[:, None, None] <= cz) & \
+                   (np.arange(nkx)[None, None, :] <= cx) & (np.arange(nj)[None, :, None] <= cy)
             amp = kk ** (-17.0 / 6.0) * mask
             phase = rng.uniform(0, 2 * np.pi, size=amp.shape)
             phi = amp * np.exp(1j * phase) * rng.standard_normal(amp.shape)
