@@ -2,29 +2,47 @@
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["api.hip", "transforms.hip", "spectral.hip", "solver.hip"]
+SOURCES = ["api.hip", "transforms.hip", "transforms_rf.hip", "spectral.hip", "solver.hip"]
 OUT = os.path.join(HERE, "libcdns_hip.so")
+OBJDIR = os.path.join(HERE, "_build")
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
-def build(verbose=False):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
-    if verbose:
-        sys.stderr.write(r.stderr)
+    return r.returncode, r.stdout + r.stderr
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source to an object (in parallel), then link the shared library."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs, cmds = [], []
+    for s in SOURCES:
+        o = os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o")
+        objs.append(o)
+        cmds.append([hipcc] + HIPCC_FLAGS + ["-c", "-o", o, os.path.join(HERE, "csrc", s)])
+    with ThreadPoolExecutor(max_workers=jobs or min(len(cmds), os.cpu_count() or 1)) as ex:
+        results = list(ex.map(_run, cmds))
+    for (rc, log), s in zip(results, SOURCES):
+        if rc != 0:
+            sys.stderr.write(log)
+            raise RuntimeError(f"hipcc failed on {s}")
+        if verbose:
+            sys.stderr.write(log)
+    tmp = OUT + ".tmp"
+    rc, log = _run([hipcc] + HIPCC_FLAGS + ["-shared", "-o", tmp] + objs)
+    if rc != 0:
+        sys.stderr.write(log)
+        raise RuntimeError("hipcc link failed")
     os.replace(tmp, OUT)
     return OUT
 

@@ -83,6 +83,7 @@ struct cdns_ctx {
     void *twz = nullptr, *twx = nullptr;
     FftPlan pz{}, px{};
     int KT = 13, P = 4;
+    int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
     std::vector<double> hWAVX, hWAVZ;
 
     // solver tables
@@ -210,6 +211,7 @@ ZArgs zargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs)
     a.NQ = c->g.NQ;
     a.KT = c->KT;
     a.NL = c->g.NL;
+    a.rf = c->rf;
     return a;
 }
 XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs, int P) {
@@ -224,6 +226,7 @@ XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs,
     a.NL = c->g.NL;
     a.NZ2 = c->g.NZ2;
     a.P = P;
+    a.rf = c->rf;
     return a;
 }
 
@@ -525,6 +528,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         if (cfg->precision != CDNS_FP64 && cfg->precision != CDNS_MIXED)
             throw std::invalid_argument("precision must be CDNS_FP64 or CDNS_MIXED");
         c->fp32 = cfg->precision == CDNS_MIXED;
+        if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;
         g.nzk = 2 * g.NZS + 1;

@@ -23,6 +23,7 @@ struct ZArgs {
     const void *tw;
     FftPlan plan;
     int nkx, nzk, NZS, NQ, KT, NL;
+    int rf; // use the register-resident FFT when available
 };
 
 // x stages (FFT over kx) and the cross product.
@@ -33,6 +34,7 @@ struct XArgs {
     const void *tw;
     FftPlan plan;
     int nkx, NL, NZ2, P;
+    int rf;
     double *phys; // reference-layout physical field for the single-field stages
 };
 
@@ -43,6 +45,13 @@ void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);
 void launch_xprod(const XArgs &a, bool fp32, hipStream_t st);
 void launch_xbwd_phys(const XArgs &a, bool fp32, hipStream_t st);
 void launch_xfwd_phys(const XArgs &a, bool fp32, hipStream_t st);
+// register-resident two-pass FFT variants; return false for lengths without an instantiation
+bool rf_supported(int n);
+bool launch_xprod_rf(const XArgs &a, bool fp32, hipStream_t st);
+bool launch_xbwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st);
+bool launch_xfwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st);
+bool launch_zbwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st);
+bool launch_zfwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st);
 
 // ----------------------------------------------------------- spectral ----
 struct LayoutArgs {

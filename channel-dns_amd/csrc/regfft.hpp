@@ -1,0 +1,156 @@
+// Register-resident small DFTs with compile-time twiddles, and the two-pass (N = N1 x N2)
+// line FFT built on them: each thread of a lane group does N2- and N1-point DFTs in
+// registers, with one LDS transpose between the passes.
+#pragma once
+
+#include "common.hpp"
+
+#include <type_traits>
+#include <utility>
+
+namespace cdns {
+
+// ------------------------------------------------- compile-time trigonometry --
+namespace ct {
+constexpr double kPiOver2 = 1.57079632679489661923132169163975144;
+
+constexpr double sin_taylor(double x) { // |x| <= pi/4
+    double x2 = x * x, term = x, sum = x;
+    for (int i = 1; i < 14; ++i) {
+        term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cos_taylor(double x) {
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int i = 1; i < 14; ++i) {
+        term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+// cos(2 pi k / N) (which = 0) or sin(2 pi k / N) (which = 1), with exact quadrant reduction.
+constexpr double trig(long long k, long long N, int which) {
+    long long m = (4 * k) % (4 * N);
+    if (m < 0) m += 4 * N;
+    const long long q = m / N, r = m - q * N; // angle = (q + r/N) pi/2
+    double c, s;
+    if (2 * r <= N) {
+        const double x = kPiOver2 * (double)r / (double)N;
+        c = cos_taylor(x);
+        s = sin_taylor(x);
+    } else {
+        const double x = kPiOver2 * (double)(N - r) / (double)N;
+        c = sin_taylor(x);
+        s = cos_taylor(x);
+    }
+    if (r == 0) { c = 1.0; s = 0.0; }
+    double cc = c, ss = s;
+    if (q == 1) { cc = -s; ss = c; }
+    else if (q == 2) { cc = -c; ss = -s; }
+    else if (q == 3) { cc = s; ss = -c; }
+    return which == 0 ? cc : ss;
+}
+} // namespace ct
+
+// exp(S * 2 pi i K / N)
+template <int K, int N, int S> struct Tw {
+    static constexpr double c = ct::trig(K, N, 0);
+    static constexpr double s = S * ct::trig(K, N, 1);
+};
+
+template <int I> using ic = std::integral_constant<int, I>;
+template <int B, int E, typename F> __device__ inline void sfor(F &&f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+template <typename R, int K, int N, int S> __device__ inline cpx<R> twmul(cpx<R> v) {
+    if constexpr (K % N == 0) return v;
+    else if constexpr ((4 * K) % N == 0) {
+        // multiples of pi/2: exact rotations
+        constexpr int q = (((4 * K) / N) % 4 + 4) % 4;
+        constexpr int qs = S > 0 ? q : (4 - q) % 4;
+        if constexpr (qs == 1) return {-v.y, v.x};
+        else if constexpr (qs == 2) return {-v.x, -v.y};
+        else return {v.y, -v.x};
+    } else {
+        constexpr R c = (R)Tw<K, N, S>::c, s = (R)Tw<K, N, S>::s;
+        return {v.x * c - v.y * s, v.x * s + v.y * c};
+    }
+}
+
+constexpr int small_factor(int n) {
+    if (n % 4 == 0 && n != 8 && n > 4) return 4;
+    for (int p : {2, 3, 5, 7})
+        if (n % p == 0) return p;
+    return n;
+}
+
+// y_m = sum_q v_q exp(S 2 pi i q m / N), in registers, fully unrolled.
+template <typename R, int N, int S> struct RDft {
+    __device__ static inline void run(cpx<R> (&v)[N]) {
+        if constexpr (N == 1) {
+            return;
+        } else if constexpr (N == 2 || N == 3 || N == 4 || N == 5 || N == 7 || N == 8) {
+            Dft<R, N>::run(v, S);
+        } else {
+            constexpr int p = small_factor(N), m = N / p;
+            cpx<R> sub[p][m];
+            sfor<0, p>([&](auto r) {
+                sfor<0, m>([&](auto j) { sub[r][j] = v[r + p * j]; });
+                RDft<R, m, S>::run(sub[r]);
+            });
+            sfor<0, m>([&](auto k) {
+                cpx<R> t[p];
+                sfor<0, p>([&](auto r) { t[r] = twmul<R, decltype(r)::value * decltype(k)::value, N, S>(sub[r][k]); });
+                Dft<R, p>::run(t, S);
+                sfor<0, p>([&](auto qq) { v[k + m * qq] = t[qq]; });
+            });
+        }
+    }
+};
+
+// Lane-group layout of the two-pass FFT of one line of N = N1*N2 points.
+//   pass A: thread t < N1 holds x[t + N1*n2], n2 < N2; DFT over n2; times w^(t*k2); to LDS T[t][k2]
+//   pass B: thread t < N2 holds T[n1][t], n1 < N1; DFT over n1 -> X[t + N2*k1], k1 < N1
+// The output of pass B (index t + N2*k1) is the pass-A input layout of the transposed plan
+// (N1' = N2, N2' = N1), so a forward FFT can follow an inverse one without data movement.
+template <int N1, int N2> struct TwoPass {
+    static constexpr int N = N1 * N2;
+    static constexpr int PA = N2 + 1;  // LDS row pitch of pass-A output (conflict-free rows)
+    static constexpr int LDS = N1 * PA; // complex entries per group buffer
+};
+
+// Pass A + transpose + pass B.  `x` holds pass-A input on threads t < N1 and receives pass-B
+// output on threads t < N2 (arrays sized max(N1, N2)).  `tw` is the N-point root table
+// (exp(-2 pi i k / N)) in LDS, `T` the group's transpose buffer.  Sync is the caller's barrier.
+template <typename R, int N1, int N2, int S, int M, typename Sync>
+__device__ inline void two_pass(cpx<R> (&x)[M], int t, const cpx<R> *tw, cpx<R> *T, Sync sync) {
+    constexpr int N = N1 * N2, PA = N2 + 1;
+    if (t < N1) {
+        cpx<R> a[N2];
+        sfor<0, N2>([&](auto j) { a[j] = x[j]; });
+        RDft<R, N2, S>::run(a);
+        sfor<0, N2>([&](auto k2) {
+            constexpr int K2 = decltype(k2)::value;
+            cpx<R> w = tw[(t * K2) % N];
+            if (S > 0) w.y = -w.y;
+            const cpx<R> v = a[K2];
+            T[t * PA + K2] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+        });
+    }
+    sync();
+    if (t < N2) {
+        cpx<R> b[N1];
+        sfor<0, N1>([&](auto n1) { b[n1] = T[n1 * PA + t]; });
+        RDft<R, N1, S>::run(b);
+        sfor<0, N1>([&](auto k1) { x[k1] = b[k1]; });
+    }
+    sync();
+}
+
+} // namespace cdns

@@ -442,6 +442,7 @@ static int items_per_thread(int nlines, const FftPlan &p) {
 }
 
 void launch_zbwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st) {
+    if (a.rf && launch_zbwd_rf(a, nfields, fp32, st)) return;
     const int it = items_per_thread(a.KT, a.plan);
     dim3 grid((a.nkx + a.KT - 1) / a.KT, a.NL, nfields);
     if (fp32) {
@@ -453,6 +454,7 @@ void launch_zbwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st) {
     }
 }
 void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st) {
+    if (a.rf && launch_zfwd_rf(a, nfields, fp32, st)) return;
     const int it = items_per_thread(a.KT, a.plan);
     dim3 grid((a.nkx + a.KT - 1) / a.KT, a.NL, nfields);
     if (fp32) {
@@ -464,6 +466,7 @@ void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st) {
     }
 }
 void launch_xprod(const XArgs &a, bool fp32, hipStream_t st) {
+    if (a.rf && launch_xprod_rf(a, fp32, st)) return;
     const int it = items_per_thread(3 * a.P, a.plan);
     if ((a.P / 2) * a.plan.n > 8 * 256) throw std::runtime_error("x-product batch too large");
     dim3 grid((a.NL * a.NZ2 + a.P - 1) / a.P);
@@ -476,6 +479,7 @@ void launch_xprod(const XArgs &a, bool fp32, hipStream_t st) {
     }
 }
 void launch_xbwd_phys(const XArgs &a, bool fp32, hipStream_t st) {
+    if (a.rf && launch_xbwd_phys_rf(a, fp32, st)) return;
     const int it = items_per_thread(a.P / 2, a.plan);
     dim3 grid((a.NL * a.NZ2 + a.P - 1) / a.P);
     if (fp32) {
@@ -485,6 +489,7 @@ void launch_xbwd_phys(const XArgs &a, bool fp32, hipStream_t st) {
     }
 }
 void launch_xfwd_phys(const XArgs &a, bool fp32, hipStream_t st) {
+    if (a.rf && launch_xfwd_phys_rf(a, fp32, st)) return;
     const int it = items_per_thread(a.P / 2, a.plan);
     dim3 grid((a.NL * a.NZ2 + a.P - 1) / a.P);
     if (fp32) {

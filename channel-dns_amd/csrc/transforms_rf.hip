@@ -1,0 +1,325 @@
+// x and z stages with register-resident two-pass FFTs (N = N1 x N2, one LDS transpose per FFT).
+//
+// One wavefront per workgroup; the wavefront is split into lane groups of G >= max(N1, N2)
+// lanes, one FFT line per group.  The x stage keeps the six synthesised physical lines of a
+// point in registers, forms u x w there and transforms the products back, so nothing but
+// the NXS+1 retained coefficients per line touches HBM.  Sizes without an instantiation use
+// the LDS Stockham kernels of transforms.hip.
+#include "internal.hpp"
+#include "regfft.hpp"
+
+namespace cdns {
+
+template <int N1, int N2> struct Geo {
+    static constexpr int N = N1 * N2;
+    static constexpr int MX = N1 > N2 ? N1 : N2;
+    static constexpr int G = MX <= 8 ? 8 : (MX <= 16 ? 16 : 32);
+    static constexpr int NG = 64 / G;
+    static constexpr int L1 = TwoPass<N1, N2>::LDS, L2 = TwoPass<N2, N1>::LDS;
+    static constexpr int LB = (L1 > L2 ? (L1 > N ? L1 : N) : (L2 > N ? L2 : N));
+};
+
+__device__ inline void wsync() { __syncthreads(); }
+
+template <typename R>
+__device__ inline cpx<R> herm_pack(const cpx<R> *A, const cpx<R> *B, int k, int N, int K) {
+    // value at slot k of the full-length array whose inverse DFT is a + i b for the real
+    // lines a, b with half spectra A, B (FFTW c2r: imaginary parts of DC/Nyquist ignored)
+    int src = k;
+    bool mirror = false;
+    if (k > K) {
+        if (N - k > K) return {R(0), R(0)};
+        mirror = true;
+        src = N - k;
+    }
+    cpx<R> av = A[src], bv = B ? B[src] : cpx<R>{R(0), R(0)};
+    if (src == 0 || 2 * src == N) { av.y = R(0); bv.y = R(0); }
+    if (mirror) { av.y = -av.y; bv.y = -bv.y; }
+    return {av.x - bv.y, av.y + bv.x};
+}
+
+template <typename R, int N>
+__device__ inline void load_roots(cpx<R> *tw, const void *src) {
+    const cpx<R> *s = reinterpret_cast<const cpx<R> *>(src);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = s[i];
+}
+
+// ------------------------------------------------------- x stage + product --
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int np = a.NL * a.NZ2, K = a.nkx - 1;
+    const int p = blockIdx.x * NG + g;
+    const bool valid = p < np;
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in);
+    cpx<R> *B = buf[g];
+    cpx<R> ph[3][MX];
+    sfor<0, 3>([&](auto line) {
+        constexpr int LN = decltype(line)::value;
+        cpx<R> x[MX];
+        const cpx<R> *A = Z + (size_t)(2 * LN) * a.in_fs + (size_t)p * a.nkx;
+        const cpx<R> *Bv = Z + (size_t)(2 * LN + 1) * a.in_fs + (size_t)p * a.nkx;
+        sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+        if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = herm_pack(A, Bv, t + N1 * decltype(n2)::value, N, K); });
+        two_pass<R, N1, N2, +1>(x, t, tw, B, wsync);
+        sfor<0, MX>([&](auto j) { ph[LN][j] = x[j]; });
+    });
+    // u x w at the N1 points n = t + N2*k1 held by this thread
+    cpx<R> F[MX], Fz[MX];
+    sfor<0, MX>([&](auto j) {
+        F[j] = {R(0), R(0)};
+        Fz[j] = {R(0), R(0)};
+    });
+    if (t < N2) {
+        sfor<0, N1>([&](auto k1) {
+            const R u = ph[0][k1].x, v = ph[0][k1].y, w = ph[1][k1].x, ox = ph[1][k1].y, oy = ph[2][k1].x,
+                    oz = ph[2][k1].y;
+            F[k1] = {v * oz - w * oy, w * ox - u * oz};
+            Fz[k1] = {u * oy - v * ox, R(0)};
+        });
+    }
+    const R half = R(0.5) / R(N);
+    cpx<R> *ZN = reinterpret_cast<cpx<R> *>(a.out);
+    // forward of (Nx + i Ny): transposed plan, input already in place
+    two_pass<R, N2, N1, -1>(F, t, tw, B, wsync);
+    if (t < N1) sfor<0, N2>([&](auto k1) { B[t + N1 * decltype(k1)::value] = F[k1]; });
+    wsync();
+    if (valid)
+        for (int k = t; k <= K; k += G) {
+            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+            const cpx<R> s = f + gm, d = f - gm;
+            ZN[(size_t)0 * a.out_fs + (size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            ZN[(size_t)1 * a.out_fs + (size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
+        }
+    wsync();
+    two_pass<R, N2, N1, -1>(Fz, t, tw, B, wsync);
+    if (t < N1) sfor<0, N2>([&](auto k1) { B[t + N1 * decltype(k1)::value] = Fz[k1]; });
+    wsync();
+    if (valid)
+        for (int k = t; k <= K; k += G) {
+            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+            const cpx<R> s = f + gm;
+            ZN[(size_t)2 * a.out_fs + (size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+        }
+}
+
+// ------------------------------------------- single-field x stages (physical) --
+// Two points share one complex line: a + i b.
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_xbwd_phys_rf(XArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int np = a.NL * a.NZ2, K = a.nkx - 1;
+    const int pa = 2 * (blockIdx.x * NG + g), pb = pa + 1;
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in);
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (t < N1 && pa < np) {
+        const cpx<R> *A = Z + (size_t)pa * a.nkx;
+        const cpx<R> *Bv = pb < np ? Z + (size_t)pb * a.nkx : nullptr;
+        sfor<0, N2>([&](auto n2) { x[n2] = herm_pack(A, Bv, t + N1 * decltype(n2)::value, N, K); });
+    }
+    two_pass<R, N1, N2, +1>(x, t, tw, buf[g], wsync);
+    if (t < N2) {
+        sfor<0, N1>([&](auto k1) {
+            const int n = t + N2 * decltype(k1)::value;
+            if (pa < np) {
+                const int l = pa / a.NZ2, iz = pa - l * a.NZ2;
+                a.phys[((size_t)iz * a.NL + l) * N + n] = double(x[k1].x);
+            }
+            if (pb < np) {
+                const int l = pb / a.NZ2, iz = pb - l * a.NZ2;
+                a.phys[((size_t)iz * a.NL + l) * N + n] = double(x[k1].y);
+            }
+        });
+    }
+}
+
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_xfwd_phys_rf(XArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int np = a.NL * a.NZ2, K = a.nkx - 1;
+    const int pa = 2 * (blockIdx.x * NG + g), pb = pa + 1;
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (t < N2) {
+        sfor<0, N1>([&](auto j) {
+            const int n = t + N2 * decltype(j)::value;
+            R re = R(0), im = R(0);
+            if (pa < np) {
+                const int l = pa / a.NZ2, iz = pa - l * a.NZ2;
+                re = R(a.phys[((size_t)iz * a.NL + l) * N + n]);
+            }
+            if (pb < np) {
+                const int l = pb / a.NZ2, iz = pb - l * a.NZ2;
+                im = R(a.phys[((size_t)iz * a.NL + l) * N + n]);
+            }
+            x[j] = {re, im};
+        });
+    }
+    two_pass<R, N2, N1, -1>(x, t, tw, buf[g], wsync);
+    cpx<R> *B = buf[g];
+    if (t < N1) sfor<0, N2>([&](auto k1) { B[t + N1 * decltype(k1)::value] = x[k1]; });
+    wsync();
+    const R half = R(0.5) / R(N);
+    cpx<R> *ZN = reinterpret_cast<cpx<R> *>(a.out);
+    for (int k = t; k <= K; k += G) {
+        const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+        const cpx<R> s = f + gm, d = f - gm;
+        if (pa < np) ZN[(size_t)pa * a.nkx + k] = {half * s.x, half * s.y};
+        if (pb < np) ZN[(size_t)pb * a.nkx + k] = {half * d.y, -half * d.x};
+    }
+}
+
+// ------------------------------------------------------------------ z stages --
+// backward: one line per group = (f, l, kx); slot k = t + N1*n2 of the kz axis.
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_zbwd_rf(ZArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int kx = blockIdx.x * NG + g, l = blockIdx.y, f = blockIdx.z;
+    const bool valid = kx < a.nkx;
+    const int NZS = a.NZS;
+    const cpx<R> *Y = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * a.NQ +
+                      (size_t)(valid ? kx : 0) * a.nzk;
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (valid && t < N1) {
+        sfor<0, N2>([&](auto n2) {
+            const int s = t + N1 * decltype(n2)::value;
+            // slot s holds kz = s (s <= NZS) or s - N (s >= N - NZS); at an overlap the slot
+            // N - NZS takes kz = -NZS (transforms.cpp:717-726)
+            int kz = 1 << 30;
+            if (s <= NZS) kz = s;
+            else if (s >= N - NZS) kz = s - N;
+            if (s == N - NZS && s <= NZS) kz = -NZS;
+            if (kz != (1 << 30)) x[n2] = Y[kz + NZS];
+        });
+    }
+    two_pass<R, N1, N2, +1>(x, t, tw, buf[g], wsync);
+    if (valid && t < N2) {
+        cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx;
+        sfor<0, N1>([&](auto k1) { Z[(size_t)(t + N2 * decltype(k1)::value) * a.nkx] = x[k1]; });
+    }
+}
+
+// forward: line (f, l, kx) over iz; keep the retained kz, x 1/NZ2.
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int kx = blockIdx.x * NG + g, l = blockIdx.y, f = blockIdx.z;
+    const bool valid = kx < a.nkx;
+    const int NZS = a.NZS;
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx +
+                      (valid ? kx : 0);
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[(size_t)(t + N1 * decltype(n2)::value) * a.nkx]; });
+    two_pass<R, N1, N2, -1>(x, t, tw, buf[g], wsync);
+    if (valid && t < N2) {
+        const R inv = R(1.0 / N);
+        cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
+        sfor<0, N1>([&](auto k1) {
+            const int s = t + N2 * decltype(k1)::value;
+            // when N = 2 NZS the slot NZS serves both kz = NZS and kz = -NZS
+            if (s <= NZS) Y[s + NZS] = inv * x[k1];
+            if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
+        });
+    }
+}
+
+// ------------------------------------------------------------------ launch --
+template <typename R, int N1, int N2> static void go_xprod(const XArgs &a, hipStream_t st) {
+    using Gm = Geo<N1, N2>;
+    hipLaunchKernelGGL((k_xprod_rf<R, N1, N2>), dim3((a.NL * a.NZ2 + Gm::NG - 1) / Gm::NG), dim3(64), 0, st, a);
+}
+template <typename R, int N1, int N2> static void go_xbwd(const XArgs &a, hipStream_t st) {
+    using Gm = Geo<N1, N2>;
+    hipLaunchKernelGGL((k_xbwd_phys_rf<R, N1, N2>), dim3((a.NL * a.NZ2 + 2 * Gm::NG - 1) / (2 * Gm::NG)), dim3(64), 0,
+                       st, a);
+}
+template <typename R, int N1, int N2> static void go_xfwd(const XArgs &a, hipStream_t st) {
+    using Gm = Geo<N1, N2>;
+    hipLaunchKernelGGL((k_xfwd_phys_rf<R, N1, N2>), dim3((a.NL * a.NZ2 + 2 * Gm::NG - 1) / (2 * Gm::NG)), dim3(64), 0,
+                       st, a);
+}
+template <typename R, int N1, int N2> static void go_zbwd(const ZArgs &a, int nf, hipStream_t st) {
+    using Gm = Geo<N1, N2>;
+    hipLaunchKernelGGL((k_zbwd_rf<R, N1, N2>), dim3((a.nkx + Gm::NG - 1) / Gm::NG, a.NL, nf), dim3(64), 0, st, a);
+}
+template <typename R, int N1, int N2> static void go_zfwd(const ZArgs &a, int nf, hipStream_t st) {
+    using Gm = Geo<N1, N2>;
+    hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3((a.nkx + Gm::NG - 1) / Gm::NG, a.NL, nf), dim3(64), 0, st, a);
+}
+
+// Supported lengths: N = N1 * N2 with N1, N2 <= 16.
+#define CDNS_RF_SIZES(X) X(196, 14, 14) X(48, 6, 8) X(96, 8, 12) X(32, 4, 8) X(64, 8, 8) X(128, 8, 16) \
+    X(144, 12, 12) X(192, 12, 16) X(256, 16, 16) X(36, 6, 6) X(72, 8, 9) X(120, 10, 12) X(160, 10, 16)
+
+bool rf_supported(int n) {
+#define X(NN, A, B) if (n == NN) return true;
+    CDNS_RF_SIZES(X)
+#undef X
+    return false;
+}
+
+bool launch_xprod_rf(const XArgs &a, bool fp32, hipStream_t st) {
+    const int n = a.plan.n;
+#define X(NN, A, B) case NN: if (fp32) go_xprod<float, A, B>(a, st); else go_xprod<double, A, B>(a, st); return true;
+    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+#undef X
+}
+bool launch_xbwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st) {
+    const int n = a.plan.n;
+#define X(NN, A, B) case NN: if (fp32) go_xbwd<float, A, B>(a, st); else go_xbwd<double, A, B>(a, st); return true;
+    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+#undef X
+}
+bool launch_xfwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st) {
+    const int n = a.plan.n;
+#define X(NN, A, B) case NN: if (fp32) go_xfwd<float, A, B>(a, st); else go_xfwd<double, A, B>(a, st); return true;
+    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+#undef X
+}
+bool launch_zbwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st) {
+    const int n = a.plan.n;
+#define X(NN, A, B) case NN: if (fp32) go_zbwd<float, A, B>(a, nf, st); else go_zbwd<double, A, B>(a, nf, st); return true;
+    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+#undef X
+}
+bool launch_zfwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st) {
+    const int n = a.plan.n;
+#define X(NN, A, B) case NN: if (fp32) go_zfwd<float, A, B>(a, nf, st); else go_zfwd<double, A, B>(a, nf, st); return true;
+    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+#undef X
+}
+
+} // namespace cdns

@@ -20,8 +20,22 @@ CONFIGS = {
 }
 
 
+# the x and z stages have two kernel families: register-resident two-pass FFTs (default for
+# the lengths they are instantiated for) and the LDS Stockham kernels (CDNS_FFT=stockham)
+FFT_KINDS = ["register", "stockham"]
+
+
+def _use_fft(monkeypatch, kind):
+    if kind == "stockham":
+        monkeypatch.setenv("CDNS_FFT", "stockham")
+    else:
+        monkeypatch.delenv("CDNS_FFT", raising=False)
+
+
+@pytest.mark.parametrize("fft", FFT_KINDS)
 @pytest.mark.parametrize("name", list(CONFIGS))
-def test_fine_transform_backward_forward(gpu, name):
+def test_fine_transform_backward_forward(gpu, monkeypatch, name, fft):
+    _use_fft(monkeypatch, fft)
     cfg = CONFIGS[name]
     dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
     spec = band_spectrum(cfg, 0xF5)
@@ -56,8 +70,10 @@ def test_strided_component_writes_only_its_slot(gpu):
     assert maxrel(out[..., 2], src) < 1e-12
 
 
+@pytest.mark.parametrize("fft", FFT_KINDS)
 @pytest.mark.parametrize("name", ["channel_32", "channel_64"])
-def test_curl_cross_nonlinear(gpu, name):
+def test_curl_cross_nonlinear(gpu, monkeypatch, name, fft):
+    _use_fft(monkeypatch, fft)
     cfg = CONFIGS[name]
     dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
     # a generic realisable field (no wall conditions), so the pressure wall term is O(1)
@@ -108,9 +124,14 @@ def test_solve_modes(gpu, correct, fuse):
     assert maxrel(V1, V2) < 1e-10
 
 
-@pytest.mark.parametrize("name,steps,tol", [("channel_32", 1, 1e-11), ("channel_32", 5, 1e-10),
-                                            ("channel_32", 20, 1e-9), ("stage3", 5, 1e-10)])
-def test_step_parity(gpu, name, steps, tol):
+@pytest.mark.parametrize("name,steps,tol,fft", [("channel_32", 1, 1e-11, "register"),
+                                                ("channel_32", 5, 1e-10, "register"),
+                                                ("channel_32", 20, 1e-9, "register"),
+                                                ("stage3", 5, 1e-10, "register"),
+                                                ("channel_32", 5, 1e-10, "stockham"),
+                                                ("stage3", 5, 1e-10, "stockham")])
+def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
+    _use_fft(monkeypatch, fft)
     cfg = stage3_config() if name == "stage3" else CONFIGS[name]
     u0 = fields.seeded_velocity(cfg, seed=5)
     dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
