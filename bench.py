@@ -3,7 +3,10 @@
 A step is one full semi-implicit time step (nonlinear term through the 3/2-dealiased
 product grid, RHS, coupled tau solves, mean-mode and flow corrections) of the
 Re_tau=180 channel: base grid 128x129x128, product grid 196x193x196, FP64.  The state
-is resident in HBM before timing starts; steps 1-2 (solver setup) are warm-up.
+is resident in HBM before timing starts; steps 1-2 (solver setup) are warm-up.  The time
+step is the reference's own for this grid (DT=5e-5, examples/Re_tau180/Re_tau180.toml:52):
+the cost of a step does not depend on dt, and at dt=1e-3 the explicitly advected highest
+kx modes grow by >10% per step, so a 100-step run of a k^-5/3 field does not stay finite.
 
 Prints one JSON line.  `roofline` is for the kernel with the largest share of step time:
 algorithmic bytes per launch (see `ALGO`) over its mean launch time from HIP events on
@@ -73,6 +76,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="retau180")
+    ap.add_argument("--dt", type=float, default=5e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -87,8 +91,8 @@ def main():
     from cdns import config, fields
 
     torch.cuda.set_device(0)
-    cfg = config.WORKLOADS[args.workload]()
-    u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1, kcut=8)
+    cfg = dict(config.WORKLOADS[args.workload](), DT=args.dt)
+    u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
     warm = max(args.warmup, 2)
 
     res = {}
@@ -103,7 +107,8 @@ def main():
         dev.sync()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        finite = bool(np.isfinite(dev.get_state("velocity")).all())
+        final = dev.get_state("velocity")
+        finite = bool(np.isfinite(final).all())
         # per-kernel times: the same steps again from the same state, with HIP events on the
         # backend's stream around every launch
         dev.set_state("velocity", u0)
@@ -114,7 +119,7 @@ def main():
         dev.sync()
         prof = dev.profile_read()
         dev.profile(False)
-        res[name] = dict(elapsed=el, prof=prof, finite=finite)
+        res[name] = dict(elapsed=el, prof=prof, finite=finite, final=final)
         dev.close()
 
     r = res["f64"]
@@ -136,6 +141,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, u0, args.cpu_seconds)
     mixed_sps = args.steps / res["mixed"]["elapsed"]
+    # the mixed run against the FP64 run from the same initial state after warm + steps steps
+    mixed_err = fields.precision_report(cfg, res["mixed"]["final"], r["final"])
     line = {
         "metric": "steps/s at 128x129x128 (product grid 196x193x196), FP64 step on one MI355X",
         "value": steps_per_s,
@@ -148,7 +155,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: seeded log-law mean + divergence-free k^-5/3 perturbation (|k|<=8, rms 10% of centreline)",
+        "data": "synthetic: seeded log-law mean + divergence-free k^-5/3 perturbation (rms 10% of centreline)",
         "config": {"workload": f"{args.workload}: base {cfg['NX']}x{cfg['NY'] + 1}x{cfg['NZ']}, product "
                                f"{cfg['NX2']}x{cfg['NY2'] + 1}x{cfg['NZ2']}, Re={cfg['Re']}, dt={cfg['DT']}",
                    "steps_timed_from_iteration": warm + 1},
@@ -156,6 +163,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": {"steps_per_s": mixed_sps, "ms_per_step": 1e3 * res["mixed"]["elapsed"] / args.steps,
                   "speedup_vs_f64": mixed_sps / steps_per_s, "finite": res["mixed"]["finite"],
+                  "error_vs_f64_after_steps": warm + args.steps, "error_vs_f64": mixed_err,
                   "kernels_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in res["mixed"]["prof"].items()}},
         "adjoint_forward_ratio": None,
         "kernels": kern,

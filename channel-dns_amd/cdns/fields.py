@@ -171,3 +171,38 @@ def rel_l2(a, b):
 
 def rel_l2_components(a, b):
     return [rel_l2(a[..., c], b[..., c]) for c in range(a.shape[-1])]
+
+
+# ------------------------------------------------------------ flow diagnostics --
+def mean_streamwise(cfg, vel):
+    """Chebyshev coefficients of the plane-averaged streamwise velocity (mode kx = kz = 0)."""
+    return np.real(vel[cfg["NZS"], :, 0, 0]).copy()
+
+
+def wall_shear(cfg, vel):
+    """Mean wall shear d<u>/dy at y = +1 and y = -1 from the Chebyshev coefficients:
+    T_j'(1) = j^2 and T_j'(-1) = (-1)^(j+1) j^2 (doc/algorithm.md:174)."""
+    c = mean_streamwise(cfg, vel)
+    j = np.arange(c.size, dtype=np.float64)
+    return float(np.sum(j * j * c)), float(np.sum((-1.0) ** (j + 1) * j * j * c))
+
+
+def divergence_l2(cfg, vel):
+    """L2 norm of the spectral divergence i kx u + dv/dy + i kz w over all retained modes."""
+    kx, kz = _wavenumbers(cfg)
+    u, v, w = vel[..., 0], vel[..., 1], vel[..., 2]
+    dv = cheb_dy(v)  # axis 1 is the Chebyshev index
+    div = 1j * kx[None, None, :] * u + dv + 1j * kz[:, None, None] * w
+    return float(np.linalg.norm(div.ravel()))
+
+
+def precision_report(cfg, got, ref):
+    """Errors of a state against a reference state: per-component velocity rel-L2, the mean
+    profile (rel-L2 of its coefficients), both wall shears (relative) and the divergence of each."""
+    sg, sr = wall_shear(cfg, got), wall_shear(cfg, ref)
+    return {
+        "velocity_rel_l2": rel_l2_components(got, ref),
+        "mean_profile_rel_l2": rel_l2(mean_streamwise(cfg, got), mean_streamwise(cfg, ref)),
+        "wall_shear_rel": [abs(a - b) / abs(b) for a, b in zip(sg, sr)],
+        "divergence_l2": [divergence_l2(cfg, got), divergence_l2(cfg, ref)],
+    }

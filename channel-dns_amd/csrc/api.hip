@@ -84,6 +84,7 @@ struct cdns_ctx {
     FftPlan pz{}, px{};
     int KT = 13, P = 4;
     int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
+    int ymfma = 1; // y stages on the matrix cores (CDNS_YSTAGE=vector selects the vector-ALU kernels)
     std::vector<double> hWAVX, hWAVZ;
 
     // solver tables
@@ -195,6 +196,7 @@ YArgs yargs(cdns_ctx *c) {
     a.NY2 = g.NY2;
     a.YS = g.YS;
     a.y = c->Ybuf;
+    a.mfma = c->ymfma;
     return a;
 }
 ZArgs zargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs) {
@@ -529,6 +531,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
             throw std::invalid_argument("precision must be CDNS_FP64 or CDNS_MIXED");
         c->fp32 = cfg->precision == CDNS_MIXED;
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
+        if (const char *e = getenv("CDNS_YSTAGE")) c->ymfma = std::string(e) != "vector";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;
         g.nzk = 2 * g.NZS + 1;

@@ -13,6 +13,8 @@ struct YArgs {
     const double *coef, *coef2;
     int nj, nje, njo, JS, NQ, NLH, NY2, nwj, ng;
     size_t YS;
+    int mfma;                   // use the matrix-core kernels when the tables fit
+    int has_mid, rows_m, njp, KL; // matrix-core shape (set by their launchers)
 };
 
 // z stages (FFT over kz).
@@ -39,6 +41,9 @@ struct XArgs {
 };
 
 void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
+// matrix-core variants; return false when a table does not fit the LDS budget
+bool launch_ybwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st);
+bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st);
 void launch_yfwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
 void launch_zbwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);
 void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);

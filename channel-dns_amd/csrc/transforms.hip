@@ -409,6 +409,7 @@ template <typename R> static void launch_ybwd_t(const YArgs &a, int nfields, hip
     hipLaunchKernelGGL((k_ybwd<R, LW>), grid, block, sm, st, a);
 }
 void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st) {
+    if (a.mfma && launch_ybwd_mfma(a, nfields, fp32, st)) return;
     if (fp32) launch_ybwd_t<float>(a, nfields, st);
     else launch_ybwd_t<double>(a, nfields, st);
 }
@@ -424,6 +425,7 @@ template <typename R> static void launch_yfwd_t(YArgs a, int nfields, hipStream_
     hipLaunchKernelGGL((k_yfwd<R, C>), grid, block, sm, st, a);
 }
 void launch_yfwd(const YArgs &a, int nfields, bool fp32, hipStream_t st) {
+    if (a.mfma && launch_yfwd_mfma(a, nfields, fp32, st)) return;
     if (fp32) launch_yfwd_t<float>(a, nfields, st);
     else launch_yfwd_t<double>(a, nfields, st);
 }

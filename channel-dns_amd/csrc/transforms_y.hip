@@ -1,0 +1,326 @@
+// y stages as dense Chebyshev matrix applies on the matrix cores (16x16x4 MFMA, FP64 or FP32).
+//
+// Backward (spectral -> Y):  Y[l][q] = sum_j B[j][l] S[q][j], split by parity of j:
+//   E[l'] = sum_{j even} B[j][l'] S_j,  O[l'] = sum_{j odd} B[j][l'] S_j,
+//   Y[l'] = E + O,  Y[NY2 - l'] = E - O  for l' < NY2/2; for even NY2 the middle row
+//   l' = NY2/2 only has even terms and is summed on the vector ALUs.
+//   MFMA: D[l'][col] += A[l'][j] * B[j][col], A = the Chebyshev table (LDS), B = the spectral
+//   data (one real column per (q, re|im), read straight from HBM).
+// Forward (Y -> spectral):  S[q][j] = sum_{l'} F_p[l'][m] (Y[l'] +- Y[NY2 - l'])[q], j = 2m + p.
+//   MFMA: D[col][m] += A[col][l'] * B[l'][m], A = the folded data, B = the table (LDS); each
+//   lane ends with the re/im pair of one (q, j) and writes it as one complex.
+// A wave owns 8*CT complex columns; its work is split over row parts (backward) or j parts
+// (forward) so that each part's table fits in 64 KB of LDS and the grid has enough waves.
+//
+// MFMA operand layout (16x16x4): A[i][k] and B[k][n] at lane i + 16k (n + 16k); D[i][n] in
+// register r of lane n + 16*(i/4), r = i % 4.
+#include "internal.hpp"
+
+#include <algorithm>
+
+namespace cdns {
+
+template <typename R> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static inline v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kLdsBudget = 64 * 1024;
+constexpr int kWaves = 8;
+
+// ------------------------------------------------------------------ backward --
+// grid: (column groups of 8 waves, row parts, fields); TRP row tiles (16 l' each) per part.
+template <typename R, int TRP, int CT>
+__global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
+    using M = Mfma<R>;
+    using v4 = typename M::v4;
+    constexpr int PITCH = TRP * 16;
+    extern __shared__ unsigned char smem_raw[];
+    R *Bt = reinterpret_cast<R *>(smem_raw); // [njp][PITCH]
+    R *midc = Bt + (size_t)a.njp * PITCH;    // [njp], middle-row coefficients
+    const int f = blockIdx.z, part = blockIdx.y;
+    const int l0 = part * PITCH;
+    const bool do_mid = a.has_mid && part == gridDim.y - 1;
+    const int NLH = a.NLH, nj = a.nj, rows = a.rows_m;
+    for (int e = threadIdx.x; e < a.njp * PITCH; e += blockDim.x) {
+        const int j = e / PITCH, lp = l0 + (e - j * PITCH);
+        Bt[e] = (j < nj && lp < rows) ? R(a.coef[(size_t)j * NLH + lp]) : R(0);
+    }
+    if (do_mid)
+        for (int j = threadIdx.x; j < a.njp; j += blockDim.x) midc[j] = j < nj ? R(a.coef[(size_t)j * NLH + NLH - 1]) : R(0);
+    __syncthreads();
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, k = lane >> 4;
+    const int qb = (blockIdx.x * kWaves + wv) * 8 * CT;
+    const double *src[CT];
+    bool okq[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int q = qb + 8 * c + (col >> 1);
+        okq[c] = q < a.NQ;
+        src[c] = reinterpret_cast<const double *>(a.spec[f]) + ((size_t)(okq[c] ? q : 0) * a.JS) * 2 + (col & 1);
+    }
+    v4 accE[CT][TRP], accO[CT][TRP];
+    R mid[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        mid[c] = R(0);
+#pragma unroll
+        for (int t = 0; t < TRP; ++t) {
+            accE[c][t] = v4{R(0), R(0), R(0), R(0)};
+            accO[c][t] = v4{R(0), R(0), R(0), R(0)};
+        }
+    }
+    // k-step s covers j = 8s .. 8s+7: this lane's even j = 8s + 2k, odd j = 8s + 2k + 1
+    const int nsteps = a.njp / 8;
+    R bE[CT], bO[CT];
+    auto load = [&](int s) {
+        const int je = 8 * s + 2 * k, jo = je + 1;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            bE[c] = (okq[c] && je < nj) ? R(src[c][2 * je]) : R(0);
+            bO[c] = (okq[c] && jo < nj) ? R(src[c][2 * jo]) : R(0);
+        }
+    };
+    load(0);
+    for (int s = 0; s < nsteps; ++s) {
+        R cE[CT], cO[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            cE[c] = bE[c];
+            cO[c] = bO[c];
+        }
+        if (s + 1 < nsteps) load(s + 1);
+        const int je = 8 * s + 2 * k;
+        const R *rowE = Bt + (size_t)je * PITCH + col, *rowO = rowE + PITCH;
+#pragma unroll
+        for (int t = 0; t < TRP; ++t) {
+            const R aE = rowE[16 * t], aO = rowO[16 * t];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                accE[c][t] = M::mma(aE, cE[c], accE[c][t]);
+                accO[c][t] = M::mma(aO, cO[c], accO[c][t]);
+            }
+        }
+        if (do_mid) {
+            const R mc = midc[je];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) mid[c] += mc * cE[c];
+        }
+    }
+    R *Y = reinterpret_cast<R *>(a.y) + (size_t)f * a.YS * 2;
+    const int NY2 = a.NY2;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int q = qb + 8 * c + (col >> 1);
+        if (do_mid) {
+            R v = mid[c];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (k == 0 && okq[c]) Y[((size_t)(NLH - 1) * a.NQ + q) * 2 + (col & 1)] = v;
+        }
+        if (!okq[c]) continue;
+#pragma unroll
+        for (int t = 0; t < TRP; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int lp = l0 + 16 * t + 4 * k + r;
+                if (lp < rows) {
+                    const R e = accE[c][t][r], o = accO[c][t][r];
+                    Y[((size_t)lp * a.NQ + q) * 2 + (col & 1)] = e + o;
+                    Y[((size_t)(NY2 - lp) * a.NQ + q) * 2 + (col & 1)] = e - o;
+                }
+            }
+    }
+}
+
+// ------------------------------------------------------------------- forward --
+// grid: (column groups of 8 waves, j parts, fields); JTP j tiles (16 m each) per parity per part.
+template <typename R, int JTP, int CT>
+__global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
+    using M = Mfma<R>;
+    using v4 = typename M::v4;
+    constexpr int PITCH = JTP * 16;
+    extern __shared__ unsigned char smem_raw[];
+    R *Fe = reinterpret_cast<R *>(smem_raw); // [KL][PITCH]
+    R *Fo = Fe + (size_t)a.KL * PITCH;       // [KL][PITCH]
+    R *Fm = Fo + (size_t)a.KL * PITCH;       // [PITCH], middle row (even j only)
+    const int f = blockIdx.z, part = blockIdx.y;
+    const int m0 = part * PITCH;
+    const int rows = a.rows_m, nje = a.nje, njo = a.njo, NLH = a.NLH;
+    for (int e = threadIdx.x; e < a.KL * PITCH; e += blockDim.x) {
+        const int lp = e / PITCH, m = m0 + (e - lp * PITCH);
+        Fe[e] = (lp < rows && m < nje) ? R(a.coef[(size_t)lp * nje + m]) : R(0);
+        Fo[e] = (lp < rows && m < njo) ? R(a.coef2[(size_t)lp * njo + m]) : R(0);
+    }
+    for (int e = threadIdx.x; e < PITCH; e += blockDim.x) {
+        const int m = m0 + e;
+        Fm[e] = (a.has_mid && m < nje) ? R(a.coef[(size_t)(NLH - 1) * nje + m]) : R(0);
+    }
+    __syncthreads();
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, k = lane >> 4;
+    const int qb = (blockIdx.x * kWaves + wv) * 8 * CT;
+    const R *Y = reinterpret_cast<const R *>(a.y) + (size_t)f * a.YS * 2;
+    const int NQ = a.NQ, NY2 = a.NY2;
+    int qa[CT];
+    bool oka[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        qa[c] = qb + 8 * c + (col >> 1);
+        oka[c] = qa[c] < NQ;
+    }
+    v4 accE[CT][JTP], accO[CT][JTP];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int t = 0; t < JTP; ++t) {
+            accE[c][t] = v4{R(0), R(0), R(0), R(0)};
+            accO[c][t] = v4{R(0), R(0), R(0), R(0)};
+        }
+    // k-step s: l' = 4s + k
+    const int nsteps = a.KL / 4;
+    R y1[CT], y2[CT];
+    auto load = [&](int s) {
+        const int lp = 4 * s + k;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const bool ok = oka[c] && lp < rows;
+            const size_t o = (size_t)qa[c] * 2 + (col & 1);
+            y1[c] = ok ? Y[(size_t)lp * NQ * 2 + o] : R(0);
+            y2[c] = ok ? Y[(size_t)(NY2 - lp) * NQ * 2 + o] : R(0);
+        }
+    };
+    load(0);
+    for (int s = 0; s < nsteps; ++s) {
+        R sv[CT], dv[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            sv[c] = y1[c] + y2[c];
+            dv[c] = y1[c] - y2[c];
+        }
+        if (s + 1 < nsteps) load(s + 1);
+        const int lp = 4 * s + k;
+        const R *rowE = Fe + (size_t)lp * PITCH + col, *rowO = Fo + (size_t)lp * PITCH + col;
+#pragma unroll
+        for (int t = 0; t < JTP; ++t) {
+            const R be = rowE[16 * t], bo = rowO[16 * t];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                accE[c][t] = M::mma(sv[c], be, accE[c][t]);
+                accO[c][t] = M::mma(dv[c], bo, accO[c][t]);
+            }
+        }
+    }
+    // this lane holds D[4k + r][16t + col]: columns 4k, 4k+1 = (re, im) of q = qb + 8c + 2k,
+    // columns 4k+2, 4k+3 of q + 1
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = qb + 8 * c + 2 * k + h;
+            if (q >= NQ) continue;
+            cpx<R> ym = {R(0), R(0)};
+            if (a.has_mid) {
+                const R *pm = Y + ((size_t)(NLH - 1) * NQ + q) * 2;
+                ym = {pm[0], pm[1]};
+            }
+            zd *out = a.spec_out[f] + (size_t)q * a.JS;
+#pragma unroll
+            for (int t = 0; t < JTP; ++t) {
+                const int m = m0 + 16 * t + col;
+                const R fm = Fm[16 * t + col];
+                if (m < nje) {
+                    const R re = accE[c][t][2 * h] + fm * ym.x, im = accE[c][t][2 * h + 1] + fm * ym.y;
+                    out[2 * m] = {double(re), double(im)};
+                }
+                if (m < njo) out[2 * m + 1] = {double(accO[c][t][2 * h]), double(accO[c][t][2 * h + 1])};
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------- launch --
+template <typename R, int TRP, int CT> static void go_ybwd(const YArgs &a, int parts, int nf, hipStream_t st) {
+    const size_t sm = ((size_t)a.njp * TRP * 16 + a.njp) * sizeof(R);
+    const int cols = 8 * CT * kWaves;
+    hipLaunchKernelGGL((k_ybwd_mfma<R, TRP, CT>), dim3((a.NQ + cols - 1) / cols, parts, nf), dim3(64 * kWaves), sm,
+                       st, a);
+}
+template <typename R, int JTP, int CT> static void go_yfwd(const YArgs &a, int parts, int nf, hipStream_t st) {
+    const size_t sm = ((size_t)2 * a.KL * JTP * 16 + JTP * 16) * sizeof(R);
+    const int cols = 8 * CT * kWaves;
+    hipLaunchKernelGGL((k_yfwd_mfma<R, JTP, CT>), dim3((a.NQ + cols - 1) / cols, parts, nf), dim3(64 * kWaves), sm,
+                       st, a);
+}
+
+static void y_shape(YArgs &a) {
+    a.has_mid = (a.NY2 % 2 == 0) ? 1 : 0;
+    a.rows_m = a.NLH - a.has_mid;
+    a.njp = (a.nj + 7) / 8 * 8;
+    a.KL = (a.rows_m + 3) / 4 * 4;
+}
+
+// largest tile count per part, at most `cap`, whose table fits the LDS budget
+static int tiles_per_part(size_t bytes_per_tile, size_t fixed, int cap) {
+    int t = cap;
+    while (t > 1 && fixed + t * bytes_per_tile > (size_t)kLdsBudget) --t;
+    return t;
+}
+
+bool launch_ybwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
+    y_shape(a);
+    const size_t esz = fp32 ? 4 : 8;
+    const int TR = (a.rows_m + 15) / 16;
+    if ((size_t)a.njp * 16 * esz + a.njp * esz > (size_t)kLdsBudget) return false;
+    const int fit = std::min(tiles_per_part((size_t)a.njp * 16 * esz, a.njp * esz, 4), TR);
+    const int parts = (TR + fit - 1) / fit;
+    const int trp = (TR + parts - 1) / parts; // balanced parts; the kernel masks rows >= rows_m
+    if (fp32) {
+        switch (trp) {
+        case 1: go_ybwd<float, 1, 2>(a, parts, nf, st); break;
+        case 2: go_ybwd<float, 2, 2>(a, parts, nf, st); break;
+        case 3: go_ybwd<float, 3, 2>(a, parts, nf, st); break;
+        default: go_ybwd<float, 4, 2>(a, parts, nf, st); break;
+        }
+    } else {
+        switch (trp) {
+        case 1: go_ybwd<double, 1, 1>(a, parts, nf, st); break;
+        case 2: go_ybwd<double, 2, 1>(a, parts, nf, st); break;
+        case 3: go_ybwd<double, 3, 1>(a, parts, nf, st); break;
+        default: go_ybwd<double, 4, 1>(a, parts, nf, st); break;
+        }
+    }
+    return true;
+}
+
+bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
+    y_shape(a);
+    const size_t esz = fp32 ? 4 : 8;
+    const int JT = (a.nje + 15) / 16;
+    if ((size_t)2 * a.KL * 16 * esz + 16 * esz > (size_t)kLdsBudget) return false;
+    const int fit = std::min(tiles_per_part((size_t)2 * a.KL * 16 * esz + 16 * esz, 0, 2), JT);
+    const int parts = (JT + fit - 1) / fit;
+    const int jtp = (JT + parts - 1) / parts;
+    if (fp32) {
+        if (jtp == 1) go_yfwd<float, 1, 2>(a, parts, nf, st);
+        else go_yfwd<float, 2, 2>(a, parts, nf, st);
+    } else {
+        if (jtp == 1) go_yfwd<double, 1, 1>(a, parts, nf, st);
+        else go_yfwd<double, 2, 1>(a, parts, nf, st);
+    }
+    return true;
+}
+
+} // namespace cdns

@@ -137,6 +137,7 @@ struct orc_ctx {
     /* simulation state */
     cplx *st[F_COUNT];
     cplx *curl_ws;
+    cplx *body; /* constant body force added to the nonlinear term, or NULL */
     double dpdx, dpdz, time;
     int setup_done;
 };
@@ -234,6 +235,7 @@ void orc_destroy(orc_ctx *c) {
     free(c->PB); free(c->VB); free(c->SIG0); free(c->u_dpdx); free(c->w_dpdz);
     for (int f = 0; f < F_COUNT; ++f) free(c->st[f]);
     free(c->curl_ws);
+    free(c->body);
     free(c);
 }
 
@@ -702,6 +704,8 @@ void orc_nonlinear_term(orc_ctx *c, double *nld, const double *veld, double *pbc
     cplx *nl = (cplx *)nld, *bc = (cplx *)pbcd;
     orc_curl(c, (double *)c->curl_ws, veld);
     orc_cross_product(c, nld, veld, (const double *)c->curl_ws);
+    if (c->body)
+        for (size_t s = 0; s < 3 * c->S; ++s) nl[s] += c->body[s];
     const double Re = c->cfg.Re;
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi) {
@@ -1042,6 +1046,9 @@ void orc_setup(orc_ctx *c, double g_v) {
     c->setup_done = 1;
 }
 
+/* g of the current velocity tables (NaN before any setup). */
+double orc_setup_g(const orc_ctx *c) { return c->setup_done ? c->g_v : NAN; }
+
 void orc_get_table(const orc_ctx *c, int which, double *out) {
     const size_t S = c->S, B = c->BND, M = c->MODE;
     switch (which) {
@@ -1131,6 +1138,16 @@ void orc_step(orc_ctx *c, int it) {
     t = c->st[F_NL_PREV]; c->st[F_NL_PREV] = c->st[F_NL]; c->st[F_NL] = t;
     t = c->st[F_PBC_PREV]; c->st[F_PBC_PREV] = c->st[F_PBC]; c->st[F_PBC] = t;
     correct_flow(c);
+}
+
+/* Constant body force f (spectral vector, reference layout) added to the nonlinear term of
+ * every step, so that it enters the RHS with weight b0 + b1 = 1 and the pressure wall
+ * condition through f_y.  Not in the reference (its only forcing is the mean gradient);
+ * used by the discrete adjoint's gradient with respect to a forcing.  NULL removes it. */
+void orc_set_body_force(orc_ctx *c, const double *f) {
+    if (!f) { free(c->body); c->body = NULL; return; }
+    if (!c->body) c->body = (cplx *)calloc(3 * c->S, sizeof(cplx));
+    memcpy(c->body, f, 3 * c->S * sizeof(cplx));
 }
 
 void orc_run(orc_ctx *c, int nsteps) {

@@ -75,6 +75,8 @@ void orc_step_coefficients(int it, double dt, double Re, double out[5]);
  * 4 P_infl0 [S cplx], 5 P_infl1, 6 V_infl0 [3S cplx], 7 V_infl1, 8 infl_div0 [bnd cplx], 9 infl_div1,
  * 10 P_b, 11 V_b, 12 sigma0 [bnd cplx], 13 u_dpdx [NYS+1], 14 w_dpdz, 15 {qx,qz} */
 void orc_get_table(const orc_ctx *c, int which, double *out);
+/* g of the current velocity tables (NaN before any setup). */
+double orc_setup_g(const orc_ctx *c);
 
 /* Simulation state (simulation.hpp:44-55). field: 0 velocity, 1 velocity_prev,
  * 2 nonlinear, 3 nonlinear_prev, 4 pressure, 5 pressure_bc, 6 pressure_bc_prev, 7 velocity_bc */
@@ -83,6 +85,8 @@ void orc_get_state(const orc_ctx *c, int field, double *data);
 /* One iteration of Simulation::run's serial body (simulation.cpp:235-247); it>=1. */
 void orc_step(orc_ctx *c, int it);
 void orc_run(orc_ctx *c, int nsteps);
+/* Constant body force added to every step's nonlinear term (no reference counterpart). */
+void orc_set_body_force(orc_ctx *c, const double *f_or_null);
 void orc_flow_gradients(const orc_ctx *c, double out[2]);
 double orc_time(const orc_ctx *c);
 
@@ -90,6 +94,24 @@ double orc_time(const orc_ctx *c);
 typedef struct { uint64_t mt[312]; int idx; } orc_mt64;
 void orc_mt64_seed(orc_mt64 *g, uint64_t seed);
 uint64_t orc_mt64_next(orc_mt64 *g);
+
+/* ---- tangent-linear and discrete adjoint of the step (cdns_adjoint.c; no reference
+ * counterpart).  A step state X is (u, u_prev, N_prev, B_prev) in reference layouts. */
+typedef struct orc_adj orc_adj;
+orc_adj *orc_adj_create(orc_ctx *c, const orc_config *cfg);
+void orc_adj_destroy(orc_adj *a);
+void orc_adj_fine_backward_T(orc_adj *a, const double *phys, double *spec);
+void orc_adj_fine_forward_T(orc_adj *a, const double *spec, double *phys);
+void orc_adj_curl_T(orc_adj *a, const double *lw, double *lu);
+/* transpose of solve_modes (fused RHS, sigma on) on its velocity output, with the tables of
+ * the context's current setup */
+void orc_adj_solve_T(orc_adj *a, const double *lv, double *lr, double *lbc);
+void orc_adj_tl_step(orc_adj *a, int it, const double *const X[4], const double *const dX[4],
+                     const double *df, double *const dXout[4]);
+void orc_adj_step(orc_adj *a, int it, const double *const X[4], double *const L[4], double *lf);
+double orc_adj_objective(orc_adj *a, int which, const double *vel, double *grad);
+double orc_adj_run(orc_adj *a, const double *u0, const double *f, int nsteps, int kc, int objective,
+                   double *grad_u0, double *grad_f);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,22 @@ def lib():
         L.orc_flow_gradients.argtypes = [vp, vp]
         L.orc_time.argtypes = [vp]
         L.orc_time.restype = d
+        L.orc_set_body_force.argtypes = [vp, vp]
+        L.orc_setup_g.argtypes = [vp]
+        L.orc_setup_g.restype = d
+        L.orc_adj_create.restype = vp
+        L.orc_adj_create.argtypes = [vp, ctypes.POINTER(OrcConfig)]
+        L.orc_adj_destroy.argtypes = [vp]
+        L.orc_adj_fine_backward_T.argtypes = [vp, vp, vp]
+        L.orc_adj_fine_forward_T.argtypes = [vp, vp, vp]
+        L.orc_adj_curl_T.argtypes = [vp, vp, vp]
+        L.orc_adj_solve_T.argtypes = [vp, vp, vp, vp]
+        L.orc_adj_tl_step.argtypes = [vp, i, vp, vp, vp, vp]
+        L.orc_adj_step.argtypes = [vp, i, vp, vp, vp]
+        L.orc_adj_objective.argtypes = [vp, i, vp, vp]
+        L.orc_adj_objective.restype = d
+        L.orc_adj_run.argtypes = [vp, vp, vp, i, i, i, vp, vp]
+        L.orc_adj_run.restype = d
         L.orc_mt64_seed.argtypes = [ctypes.POINTER(Mt64), ctypes.c_uint64]
         L.orc_mt64_next.argtypes = [ctypes.POINTER(Mt64)]
         L.orc_mt64_next.restype = ctypes.c_uint64
@@ -91,6 +107,7 @@ class Oracle:
         for j in range(3):
             c.a_V[j] = float(cfg["a_V"][j])
             c.b_V[j] = float(cfg["b_V"][j])
+        self._c = c
         self._h = lib().orc_create(ctypes.byref(c))
         if not self._h:
             raise ValueError(lib().orc_last_error().decode())
@@ -248,6 +265,17 @@ class Oracle:
     def run(self, nsteps):
         lib().orc_run(self._h, int(nsteps))
 
+    def set_body_force(self, f):
+        """Constant body force added to every step's nonlinear term (None removes it)."""
+        if f is None:
+            lib().orc_set_body_force(self._h, None)
+        else:
+            f = np.ascontiguousarray(f, np.complex128)
+            lib().orc_set_body_force(self._h, _p(f))
+
+    def setup_g(self):
+        return lib().orc_setup_g(self._h)
+
     def flow_gradients(self):
         out = np.zeros(2)
         lib().orc_flow_gradients(self._h, _p(out))
@@ -258,6 +286,89 @@ class Oracle:
         out = np.zeros(5)
         lib().orc_step_coefficients(int(it), float(dt), float(Re), _p(out))
         return out
+
+
+def _ptrs(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class Adjoint:
+    """Tangent-linear and discrete adjoint of the oracle's step (cdns_adjoint.c).
+
+    A step state is the list [u, u_prev, N_prev, B_prev] (vector, vector, vector, boundary)."""
+
+    def __init__(self, oracle):
+        self.o = oracle
+        self._h = lib().orc_adj_create(oracle._h, ctypes.byref(oracle._c))
+
+    def close(self):
+        if self._h:
+            lib().orc_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def state_like(self):
+        o = self.o
+        return [o.vector(), o.vector(), o.vector(), o.boundary()]
+
+    def fine_backward_T(self, phys):
+        phys = np.ascontiguousarray(phys, np.float64)
+        out = self.o.scalar()
+        lib().orc_adj_fine_backward_T(self._h, _p(phys), _p(out))
+        return out
+
+    def fine_forward_T(self, spec):
+        spec = np.ascontiguousarray(spec, np.complex128)
+        out = self.o.fine_phys()
+        lib().orc_adj_fine_forward_T(self._h, _p(spec), _p(out))
+        return out
+
+    def curl_T(self, lw):
+        lw = np.ascontiguousarray(lw, np.complex128)
+        out = self.o.vector()
+        lib().orc_adj_curl_T(self._h, _p(lw), _p(out))
+        return out
+
+    def solve_T(self, lv):
+        lv = np.ascontiguousarray(lv, np.complex128)
+        lr, lbc = self.o.vector(), self.o.boundary()
+        lib().orc_adj_solve_T(self._h, _p(lv), _p(lr), _p(lbc))
+        return lr, lbc
+
+    def tl_step(self, it, X, dX, df=None):
+        X = [np.ascontiguousarray(a, np.complex128) for a in X]
+        dX = [np.ascontiguousarray(a, np.complex128) for a in dX]
+        out = self.state_like()
+        dfa = None if df is None else np.ascontiguousarray(df, np.complex128)
+        lib().orc_adj_tl_step(self._h, int(it), _ptrs(X), _ptrs(dX), _p(dfa), _ptrs(out))
+        return out
+
+    def adj_step(self, it, X, L, lf=None):
+        """Returns the adjoint of the state before step `it` (L is not modified) and lf + df."""
+        X = [np.ascontiguousarray(a, np.complex128) for a in X]
+        L = [np.array(a, np.complex128) for a in L]
+        lf = self.o.vector() if lf is None else np.array(lf, np.complex128)
+        lib().orc_adj_step(self._h, int(it), _ptrs(X), _ptrs(L), _p(lf))
+        return L, lf
+
+    def objective(self, which, vel):
+        vel = np.ascontiguousarray(vel, np.complex128)
+        g = self.o.vector()
+        J = lib().orc_adj_objective(self._h, int(which), _p(vel), _p(g))
+        return J, g
+
+    def run(self, u0, f, nsteps, kc, objective):
+        """J(u(T)) and its gradients with respect to u0 and the body force f (None: zero force)."""
+        u0 = np.ascontiguousarray(u0, np.complex128)
+        fa = None if f is None else np.ascontiguousarray(f, np.complex128)
+        gu, gf = self.o.vector(), self.o.vector()
+        J = lib().orc_adj_run(self._h, _p(u0), _p(fa), int(nsteps), int(kc), int(objective), _p(gu), _p(gf))
+        return J, gu, gf
 
 
 def set_threads(n):

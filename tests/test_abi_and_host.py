@@ -120,3 +120,17 @@ def test_log_mean_profile_vanishes_at_walls():
     c, uc = fields.mean_profile(cfg, "log")
     assert abs(c.sum()) < 1e-12 * uc and abs((c * (-1.0) ** np.arange(c.size)).sum()) < 1e-12 * uc
     assert 15 < uc < 25
+
+
+def test_flow_diagnostics_closed_forms():
+    """Laminar u = (Re/2)(1 - y^2): du/dy = -Re at y = +1 and +Re at y = -1; a curl field is
+    divergence-free; the precision report of a state against itself is zero."""
+    from cdns import config, fields
+
+    cfg = config.channel_32()
+    u = fields.seeded_velocity(cfg, seed=3, mean="laminar", kmax=4)
+    top, bottom = fields.wall_shear(cfg, u)
+    assert abs(top + cfg["Re"]) < 1e-12 * cfg["Re"] and abs(bottom - cfg["Re"]) < 1e-12 * cfg["Re"]
+    assert fields.divergence_l2(cfg, u) < 1e-12 * np.linalg.norm(u)
+    rep = fields.precision_report(cfg, u, u)
+    assert max(rep["velocity_rel_l2"]) == 0 and rep["mean_profile_rel_l2"] == 0

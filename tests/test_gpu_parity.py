@@ -20,16 +20,19 @@ CONFIGS = {
 }
 
 
-# the x and z stages have two kernel families: register-resident two-pass FFTs (default for
-# the lengths they are instantiated for) and the LDS Stockham kernels (CDNS_FFT=stockham)
+# Two kernel families per stage: the defaults (register-resident two-pass FFTs for x and z,
+# matrix-core y stages) and the alternates (LDS Stockham FFTs, CDNS_FFT=stockham; vector-ALU
+# y stages, CDNS_YSTAGE=vector).
 FFT_KINDS = ["register", "stockham"]
 
 
 def _use_fft(monkeypatch, kind):
     if kind == "stockham":
         monkeypatch.setenv("CDNS_FFT", "stockham")
+        monkeypatch.setenv("CDNS_YSTAGE", "vector")
     else:
         monkeypatch.delenv("CDNS_FFT", raising=False)
+        monkeypatch.delenv("CDNS_YSTAGE", raising=False)
 
 
 @pytest.mark.parametrize("fft", FFT_KINDS)
