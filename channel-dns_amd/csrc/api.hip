@@ -229,6 +229,7 @@ XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs,
     a.NZ2 = c->g.NZ2;
     a.P = P;
     a.rf = c->rf;
+    for (int i = 0; i < 6; ++i) a.fmap[i] = i;
     return a;
 }
 

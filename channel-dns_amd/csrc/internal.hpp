@@ -38,6 +38,7 @@ struct XArgs {
     int nkx, NL, NZ2, P;
     int rf;
     double *phys; // reference-layout physical field for the single-field stages
+    int fmap[6];  // product: field indices (at in + fmap[i]*in_fs) of a0 a1 a2 b0 b1 b2; out = a x b
 };
 
 void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);

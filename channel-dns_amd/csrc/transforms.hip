@@ -248,7 +248,7 @@ __device__ inline cpx<R> pack_c2r_single(const cpx<R> *A, int k, int N, int K) {
     return a;
 }
 
-// One block: P points (l, iz); fields 0..5 = u, v, w, wx, wy, wz.
+// One block: P points (l, iz); input fields fmap[0..5] = a (3 components), b (3 components).
 template <typename R, int IT>
 __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
     extern __shared__ unsigned char smem_raw[];
@@ -264,8 +264,8 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
         const int pt = p0 + pp;
         cpx<R> v = {R(0), R(0)};
         if (pt < np) {
-            const cpx<R> *A = Z + (size_t)(2 * gsel) * a.in_fs + (size_t)pt * a.nkx;
-            const cpx<R> *B = Z + (size_t)(2 * gsel + 1) * a.in_fs + (size_t)pt * a.nkx;
+            const cpx<R> *A = Z + (size_t)a.fmap[2 * gsel] * a.in_fs + (size_t)pt * a.nkx;
+            const cpx<R> *B = Z + (size_t)a.fmap[2 * gsel + 1] * a.in_fs + (size_t)pt * a.nkx;
             v = pack_c2r(A, B, k, N, K);
         }
         buf[e] = v;

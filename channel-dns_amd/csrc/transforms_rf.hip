@@ -63,8 +63,8 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
     sfor<0, 3>([&](auto line) {
         constexpr int LN = decltype(line)::value;
         cpx<R> x[MX];
-        const cpx<R> *A = Z + (size_t)(2 * LN) * a.in_fs + (size_t)p * a.nkx;
-        const cpx<R> *Bv = Z + (size_t)(2 * LN + 1) * a.in_fs + (size_t)p * a.nkx;
+        const cpx<R> *A = Z + (size_t)a.fmap[2 * LN] * a.in_fs + (size_t)p * a.nkx;
+        const cpx<R> *Bv = Z + (size_t)a.fmap[2 * LN + 1] * a.in_fs + (size_t)p * a.nkx;
         sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
         if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = herm_pack(A, Bv, t + N1 * decltype(n2)::value, N, K); });
         two_pass<R, N1, N2, +1>(x, t, tw, B, wsync);

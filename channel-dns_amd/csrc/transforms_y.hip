@@ -12,26 +12,39 @@
 // A wave owns 8*CT complex columns; its work is split over row parts (backward) or j parts
 // (forward) so that each part's table fits in 64 KB of LDS and the grid has enough waves.
 //
-// MFMA operand layout (16x16x4): A[i][k] and B[k][n] at lane i + 16k (n + 16k); D[i][n] in
-// register r of lane n + 16*(i/4), r = i % 4.
+// MFMA operand layout (16x16x4): A[i][k] at lane i + 16k, B[k][n] at lane n + 16k.  The
+// result D[i][n] sits in lane n + 16*kk, register r with i = 4*kk + r for FP32 and
+// i = 4*r + kk for FP64 (Mfma<R>::row).  The forward kernel orders its data columns so that
+// each lane ends with (re, im) pairs of the same column in registers (0,1) and (2,3).
 #include "internal.hpp"
 
 #include <algorithm>
 
 namespace cdns {
 
+// row(kk, r): row index i of D held in register r of a lane with lane / 16 = kk.
+// qcol/pcol: complex column (of 8) and part (0 re, 1 im) of data column c (forward kernel);
+// qout(kk, h): the complex column whose (re, im) a lane holds in registers (2h, 2h + 1).
 template <typename R> struct Mfma;
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     __device__ static inline v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int kk, int r) { return 4 * r + kk; }
+    __device__ static inline int qcol(int c) { return (c & 3) + 4 * (c >> 3); }
+    __device__ static inline int pcol(int c) { return (c >> 2) & 1; }
+    __device__ static inline int qout(int kk, int h) { return kk + 4 * h; }
 };
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     __device__ static inline v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int kk, int r) { return 4 * kk + r; }
+    __device__ static inline int qcol(int c) { return c >> 1; }
+    __device__ static inline int pcol(int c) { return c & 1; }
+    __device__ static inline int qout(int kk, int h) { return 2 * kk + h; }
 };
 
 constexpr int kLdsBudget = 64 * 1024;
@@ -134,7 +147,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
         for (int t = 0; t < TRP; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int lp = l0 + 16 * t + 4 * k + r;
+                const int lp = l0 + 16 * t + M::row(k, r);
                 if (lp < rows) {
                     const R e = accE[c][t][r], o = accO[c][t][r];
                     Y[((size_t)lp * a.NQ + q) * 2 + (col & 1)] = e + o;
@@ -176,9 +189,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
     const int NQ = a.NQ, NY2 = a.NY2;
     int qa[CT];
     bool oka[CT];
+    const int pa = M::pcol(col);
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
-        qa[c] = qb + 8 * c + (col >> 1);
+        qa[c] = qb + 8 * c + M::qcol(col);
         oka[c] = qa[c] < NQ;
     }
     v4 accE[CT][JTP], accO[CT][JTP];
@@ -197,7 +211,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const bool ok = oka[c] && lp < rows;
-            const size_t o = (size_t)qa[c] * 2 + (col & 1);
+            const size_t o = (size_t)qa[c] * 2 + pa;
             y1[c] = ok ? Y[(size_t)lp * NQ * 2 + o] : R(0);
             y2[c] = ok ? Y[(size_t)(NY2 - lp) * NQ * 2 + o] : R(0);
         }
@@ -223,13 +237,13 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
             }
         }
     }
-    // this lane holds D[4k + r][16t + col]: columns 4k, 4k+1 = (re, im) of q = qb + 8c + 2k,
-    // columns 4k+2, 4k+3 of q + 1
+    // this lane holds (re, im) of complex column qout(k, h) in registers (2h, 2h + 1), for
+    // m = m0 + 16t + col
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int q = qb + 8 * c + 2 * k + h;
+            const int q = qb + 8 * c + M::qout(k, h);
             if (q >= NQ) continue;
             cpx<R> ym = {R(0), R(0)};
             if (a.has_mid) {
