@@ -27,7 +27,8 @@ ABI_FUNCTIONS = [
     "cdns_create", "cdns_destroy", "cdns_last_error", "cdns_sync", "cdns_stream", "cdns_xyzfft2", "cdns_curl",
     "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup", "cdns_solve_modes", "cdns_get_table",
     "cdns_set_state", "cdns_get_state", "cdns_step", "cdns_run", "cdns_flow_gradients", "cdns_time",
-    "cdns_profile", "cdns_profile_read", "cdns_profile_reset",
+    "cdns_profile", "cdns_profile_read", "cdns_profile_reset", "cdns_set_body_force", "cdns_objective",
+    "cdns_adjoint_step", "cdns_adjoint_run",
 ]
 
 
@@ -73,9 +74,14 @@ def lib(path=None):
         L.cdns_profile.argtypes = [vp, i]
         L.cdns_profile_read.argtypes = [vp, cp, vp, vp, i]
         L.cdns_profile_reset.argtypes = [vp]
+        L.cdns_set_body_force.argtypes = [vp, vp, i]
+        L.cdns_objective.argtypes = [vp, i, vp, vp, vp, i]
+        L.cdns_adjoint_step.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, i]
+        L.cdns_adjoint_run.argtypes = [vp, vp, vp, i, i, i, vp, vp, vp, i]
         for n in ("cdns_xyzfft2", "cdns_curl", "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup",
                   "cdns_solve_modes", "cdns_get_table", "cdns_set_state", "cdns_get_state", "cdns_step",
-                  "cdns_run", "cdns_flow_gradients", "cdns_profile", "cdns_profile_read", "cdns_profile_reset"):
+                  "cdns_run", "cdns_flow_gradients", "cdns_profile", "cdns_profile_read", "cdns_profile_reset",
+                  "cdns_set_body_force", "cdns_objective", "cdns_adjoint_step", "cdns_adjoint_run"):
             getattr(L, n).restype = i
         _LIB = L
     return _LIB
@@ -250,6 +256,46 @@ class Channel:
 
     def time(self):
         return lib().cdns_time(self._h)
+
+    # discrete adjoint (no reference counterpart)
+    def set_body_force(self, force):
+        """Constant body force added to every step's nonlinear term (None removes it)."""
+        if force is None:
+            self._check(lib().cdns_set_body_force(self._h, None, HOST))
+        elif isinstance(force, np.ndarray):
+            f = np.ascontiguousarray(force, np.complex128)
+            self._check(lib().cdns_set_body_force(self._h, _ptr(f), HOST))
+        else:
+            self._check(lib().cdns_set_body_force(self._h, _ptr(force), DEVICE))
+
+    def objective(self, which, velocity):
+        """(J, dJ/du) of a velocity: which 0 kinetic energy, 1 mean wall shear."""
+        u = np.ascontiguousarray(velocity, np.complex128)
+        g = self.vector()
+        J = ctypes.c_double()
+        self._check(lib().cdns_objective(self._h, int(which), _ptr(u), _ptr(g), ctypes.byref(J), HOST))
+        return J.value, g
+
+    def adjoint_step(self, it, velocity, L, lf=None):
+        """Transpose of step `it` at `velocity`: L = [adj u, adj u_prev, adj N_prev, adj B_prev] of
+        the state after the step -> those of the state before it; lf accumulates the force gradient."""
+        u = np.ascontiguousarray(velocity, np.complex128)
+        out = [np.array(a, np.complex128) for a in L]
+        f = None if lf is None else np.array(lf, np.complex128)
+        self._check(lib().cdns_adjoint_step(self._h, int(it), _ptr(u), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                            _ptr(out[3]), _ptr(f), HOST))
+        return out, f
+
+    def adjoint_run(self, u0, force, nsteps, k_ckpt, objective=0):
+        """J(u(T)) and its gradients w.r.t. u0 and the body force (None: no force, zero gradient)."""
+        u = np.ascontiguousarray(u0, np.complex128)
+        f = None if force is None else np.ascontiguousarray(force, np.complex128)
+        gu, gf = self.vector(), self.vector()
+        J = ctypes.c_double()
+        self._check(lib().cdns_adjoint_run(self._h, _ptr(u), _ptr(f), int(nsteps), int(k_ckpt), int(objective),
+                                           _ptr(gu), _ptr(gf) if f is not None else None, ctypes.byref(J), HOST))
+        self.it = nsteps
+        return J.value, gu, gf
 
     # per-kernel HIP-event timing
     def profile(self, enable=True):

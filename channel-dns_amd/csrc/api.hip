@@ -91,6 +91,10 @@ struct cdns_ctx {
     double g_v = 0.0, pa = 0.0, pb = 0.0;
     double *ad_p = nullptr, *ad_v = nullptr, *wn_p = nullptr, *wn_v = nullptr;
     zd *PI[2] = {}, *VI[2][3] = {}, *ID[2] = {}, *PB = nullptr, *VB[3] = {}, *SIG0 = nullptr;
+    // compact real tables (PIr0 PIr1 VIv0 VIv1 VIf0 VIf1 PBr VBv VBf) when u and w share wall
+    // conditions (CDNS_SOLVER_TABLES=complex keeps the complex ones in the step)
+    bool compact = false;
+    double *ctab[9] = {};
     double *u_unit = nullptr, *w_unit = nullptr, *q_unit = nullptr;
     bool setup_done = false;
 
@@ -99,6 +103,17 @@ struct cdns_ctx {
     zd *bc_prev = nullptr, *bc_ext = nullptr, *bcw = nullptr, *vbc = nullptr;
     double *flow = nullptr;
     double time = 0.0;
+
+    // constant body force added to the nonlinear term (3 planes), when set
+    zd *body[3] = {};
+    bool body_on = false;
+
+    // adjoint workspaces (allocated on first use; FP64 only)
+    bool adj_ready = false;
+    zd *a_lu[3] = {}, *a_lup[3] = {}, *a_lnp[3] = {}, *a_lbp = nullptr, *a_lf[3] = {};
+    zd *a_lNs[3] = {}, *a_lw[3] = {}, *a_lUs[3] = {}, *a_lWs[3] = {};
+    void *ZL = nullptr, *ZNa = nullptr;
+    double *Jpart = nullptr;
 
     // workspaces
     void *Ybuf = nullptr, *Zbuf = nullptr, *ZNbuf = nullptr;
@@ -229,12 +244,13 @@ XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs,
     a.NZ2 = c->g.NZ2;
     a.P = P;
     a.rf = c->rf;
-    for (int i = 0; i < 6; ++i) a.fmap[i] = i;
+    for (int i = 0; i < 6; ++i) a.fin[i] = in ? (const char *)in + i * in_fs * c->esz : nullptr;
+    for (int i = 0; i < 3; ++i) a.fout[i] = out ? (char *)out + i * out_fs * c->esz : nullptr;
     return a;
 }
 
-// spectral planes -> Y fields (n fields) -> Z fields
-void backward_yz(cdns_ctx *c, const zd *const *planes, int n) {
+// spectral planes -> Y fields (n fields) -> Z fields (the context's Z buffer, or zout)
+void backward_yz(cdns_ctx *c, const zd *const *planes, int n, void *zout = nullptr) {
     YArgs a = yargs(c);
     for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
     {
@@ -243,34 +259,37 @@ void backward_yz(cdns_ctx *c, const zd *const *planes, int n) {
     }
     {
         Launch L(c, "z_backward");
-        launch_zbwd(zargs(c, c->Ybuf, c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
+        launch_zbwd(zargs(c, c->Ybuf, zout ? zout : c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
     }
 }
-// ZN fields (n) -> Y fields -> spectral planes
-void forward_zy(cdns_ctx *c, zd *const *planes, int n) {
+// ZN fields (n, the context's ZN buffer or zin) -> Y fields -> spectral planes (+ add planes)
+void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr, zd *const *add = nullptr) {
     {
         Launch L(c, "z_forward");
-        launch_zfwd(zargs(c, c->ZNbuf, c->Ybuf, c->g.ZS, c->g.YS), n, c->fp32, c->st);
+        launch_zfwd(zargs(c, zin ? zin : c->ZNbuf, c->Ybuf, c->g.ZS, c->g.YS), n, c->fp32, c->st);
     }
     YArgs a = yargs(c);
     a.coef = c->ycoef_fe;
     a.coef2 = c->ycoef_fo;
-    for (int f = 0; f < n; ++f) a.spec_out[f] = planes[f];
+    for (int f = 0; f < n; ++f) {
+        a.spec_out[f] = planes[f];
+        a.add[f] = add ? add[f] : nullptr;
+    }
     {
         Launch L(c, "y_forward");
         launch_yfwd(a, n, c->fp32, c->st);
     }
 }
 
-// N = a x b for spectral vector planes (a: 3 planes, b: 3 planes)
-void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out) {
+// N = a x b (+ add) for spectral vector planes (a: 3 planes, b: 3 planes)
+void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out, zd *const *add = nullptr) {
     const zd *f6[6] = {a[0], a[1], a[2], b[0], b[1], b[2]};
     backward_yz(c, f6, 6);
     {
         Launch L(c, "x_product");
         launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
     }
-    forward_zy(c, out, 3);
+    forward_zy(c, out, 3, nullptr, add);
 }
 
 void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
@@ -294,7 +313,7 @@ void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
 // TimeStepper::enqueue_nonlinear_term on the context's velocity planes
 void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
     curl_planes(c, u, c->om, c->bcw);
-    cross_planes(c, u, c->om, nl);
+    cross_planes(c, u, c->om, nl, c->body_on ? c->body : nullptr);
 }
 
 // ---------------------------------------------------------------- solver --
@@ -345,6 +364,18 @@ SolveArgs solve_args(cdns_ctx *c, int mode) {
     }
     a.vbc = c->vbc;
     a.flow = c->flow;
+    a.compact = c->compact ? 1 : 0;
+    if (c->compact) {
+        a.PIr[0] = c->ctab[0];
+        a.PIr[1] = c->ctab[1];
+        a.VIv[0] = c->ctab[2];
+        a.VIv[1] = c->ctab[3];
+        a.VIf[0] = c->ctab[4];
+        a.VIf[1] = c->ctab[5];
+        a.PBr = c->ctab[6];
+        a.VBv = c->ctab[7];
+        a.VBf = c->ctab[8];
+    }
     return a;
 }
 
@@ -381,6 +412,12 @@ void setup(cdns_ctx *c, double g_v) {
         a.sig_out = c->SIG0;
         Launch L(c, "setup_bproblem");
         launch_solve(a, g.E, c->st);
+    }
+    if (c->compact) {
+        SolveArgs a = solve_args(c, MODE_STEP);
+        Launch L(c, "setup_compact");
+        launch_compact_tables(a, c->ctab[0], c->ctab[1], c->ctab[2], c->ctab[3], c->ctab[4], c->ctab[5], c->ctab[6],
+                              c->ctab[7], c->ctab[8], c->st);
     }
     c->pa = c->cfg.a_pre0;
     c->pb = c->cfg.b_pre0;
@@ -437,6 +474,176 @@ void step(cdns_ctx *c, int it) {
         std::swap(c->u[k], c->up[k]);
         std::swap(c->nl[k], c->nlp[k]);
     }
+}
+
+// ------------------------------------------------ discrete adjoint (new) --
+// The reference has no adjoint; this transposes the step above (FP64 contexts only).
+double g_for_iteration(const cdns_ctx *c, int it) {
+    double co[5];
+    step_coefficients(it < 2 ? it : 2, c->cfg.DT, c->cfg.Re, co);
+    return co[4];
+}
+
+// Make the solver tables those of g (setup runs only when they are not).
+void ensure_tables(cdns_ctx *c, double g) {
+    if (!c->setup_done || c->g_v != g) setup(c, g);
+}
+
+void ensure_adjoint(cdns_ctx *c) {
+    if (c->fp32) throw std::logic_error("the discrete adjoint runs on FP64 contexts");
+    if (c->adj_ready) return;
+    const Geometry &g = c->g;
+    for (int k = 0; k < 3; ++k) {
+        c->a_lu[k] = c->alloc<zd>(g.S);
+        c->a_lup[k] = c->alloc<zd>(g.S);
+        c->a_lnp[k] = c->alloc<zd>(g.S);
+        c->a_lf[k] = c->alloc<zd>(g.S);
+        c->a_lNs[k] = c->alloc<zd>(g.S);
+        c->a_lw[k] = c->alloc<zd>(g.S);
+        c->a_lUs[k] = c->alloc<zd>(g.S);
+        c->a_lWs[k] = c->alloc<zd>(g.S);
+    }
+    c->a_lbp = c->alloc<zd>((size_t)g.NQ * 2);
+    c->ZL = c->alloc<unsigned char>(3 * g.ZS * c->esz);
+    c->ZNa = c->alloc<unsigned char>(6 * g.ZS * c->esz);
+    c->Jpart = c->alloc<double>(g.NQ);
+    c->adj_ready = true;
+}
+
+AdjArgs adj_args(cdns_ctx *c) {
+    AdjArgs a{};
+    const Geometry &g = c->g;
+    a.ny = g.NYS;
+    a.JS = g.JS;
+    a.NQ = g.NQ;
+    a.nzk = g.nzk;
+    a.nkx = g.nkx;
+    a.NX2 = g.NX2;
+    a.NY2 = g.NY2;
+    a.collide = (g.NZ2 - g.NZS <= g.NZS) ? 1 : 0;
+    a.WAVX = c->WAVX;
+    a.WAVZ = c->WAVZ;
+    a.WL = c->WL;
+    a.WD = c->WD;
+    a.WR = c->WR;
+    a.Re = c->cfg.Re;
+    a.G_pre = c->cfg.G_pre;
+    a.g_v = c->g_v;
+    a.pa = c->pa;
+    a.pb = c->pb;
+    a.XL = c->cfg.XL;
+    a.ZL = c->cfg.ZL;
+    for (int k = 0; k < 3; ++k) {
+        a.aV[k] = c->cfg.a_V[k];
+        a.bV[k] = c->cfg.b_V[k];
+    }
+    a.grad_pre = c->cfg.INDEX_GRAD_PRE;
+    a.ad_p = c->ad_p;
+    a.ad_v = c->ad_v;
+    for (int i = 0; i < 2; ++i) {
+        a.PI[i] = c->PI[i];
+        a.ID[i] = c->ID[i];
+        for (int k = 0; k < 3; ++k) a.VI[i][k] = c->VI[i][k];
+    }
+    for (int k = 0; k < 3; ++k) a.VB[k] = c->VB[k];
+    a.SIG0 = c->SIG0;
+    a.u_unit = c->u_unit;
+    a.w_unit = c->w_unit;
+    a.q_unit = c->q_unit;
+    for (int k = 0; k < 3; ++k) {
+        a.lu[k] = c->a_lu[k];
+        a.lup[k] = c->a_lup[k];
+        a.lnp[k] = c->a_lnp[k];
+        a.lf[k] = c->a_lf[k];
+        a.lNs[k] = c->a_lNs[k];
+        a.lw[k] = c->a_lw[k];
+        a.lUs[k] = c->a_lUs[k];
+        a.lWs[k] = c->a_lWs[k];
+    }
+    a.lbp = c->a_lbp;
+    a.Jpart = c->Jpart;
+    return a;
+}
+
+// Adjoint of step `it` on the adjoint planes (a_lu, a_lup, a_lnp, a_lbp), accumulating the
+// body-force gradient into a_lf when with_force.  Zuw holds the step's synthesised velocity and
+// curl in the Z layout (six fields: u, v, w, wx, wy, wz), as the forward step left them.
+void adjoint_step(cdns_ctx *c, int it, const void *Zuw, bool with_force) {
+    const Geometry &g = c->g;
+    double co[5];
+    step_coefficients(it, c->cfg.DT, c->cfg.Re, co);
+    ensure_tables(c, g_for_iteration(c, it));
+    AdjArgs a = adj_args(c);
+    a.a0 = co[0];
+    a.a1 = co[1];
+    a.b0 = co[2];
+    a.b1 = co[3];
+    if (!with_force)
+        for (int k = 0; k < 3; ++k) a.lf[k] = nullptr;
+    {
+        Launch L(c, "adj_solve");
+        launch_adj_solve(a, g.E, c->st);
+    }
+    if (a.collide) {
+        Launch L(c, "adj_fold");
+        launch_adj_fold(a, c->st);
+    }
+    // transposed nonlinear term: Lambda = synthesis of the weighted adjoint, then
+    // lU = W x Lambda and lW = Lambda x U, analysed back
+    backward_yz(c, c->a_lNs, 3, c->ZL);
+    const size_t fs = g.ZS * c->esz;
+    auto fld = [&](const void *base, int i) { return (const void *)((const char *)base + i * fs); };
+    {
+        XArgs x = xargs(c, nullptr, nullptr, 0, 0, c->P);
+        for (int i = 0; i < 3; ++i) {
+            x.fin[i] = fld(Zuw, 3 + i);
+            x.fin[3 + i] = fld(c->ZL, i);
+            x.fout[i] = (void *)fld(c->ZNa, i);
+        }
+        Launch L(c, "adj_x_product");
+        launch_xprod(x, c->fp32, c->st);
+    }
+    {
+        XArgs x = xargs(c, nullptr, nullptr, 0, 0, c->P);
+        for (int i = 0; i < 3; ++i) {
+            x.fin[i] = fld(c->ZL, i);
+            x.fin[3 + i] = fld(Zuw, i);
+            x.fout[i] = (void *)fld(c->ZNa, 3 + i);
+        }
+        Launch L(c, "adj_x_product");
+        launch_xprod(x, c->fp32, c->st);
+    }
+    zd *outs[6] = {c->a_lUs[0], c->a_lUs[1], c->a_lUs[2], c->a_lWs[0], c->a_lWs[1], c->a_lWs[2]};
+    forward_zy(c, outs, 6, c->ZNa);
+    {
+        Launch L(c, "adj_combine");
+        launch_adj_combine(a, g.E, c->st);
+    }
+}
+
+// Terminal objective of the velocity planes u: gradient into grad planes, J returned.
+double objective(cdns_ctx *c, int which, zd *const *u, zd *const *grad) {
+    if (which != 0 && which != 1) throw std::invalid_argument("objective: 0 kinetic energy, 1 mean wall shear");
+    AdjArgs a = adj_args(c);
+    a.which = which;
+    for (int k = 0; k < 3; ++k) {
+        a.lu[k] = u[k];
+        a.grad[k] = grad[k];
+    }
+    {
+        Launch L(c, "objective");
+        launch_objective(a, c->g.E, c->st);
+    }
+    std::vector<double> h(c->g.NQ);
+    CDNS_HIP(hipMemcpyAsync(h.data(), c->Jpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    CDNS_HIP(hipStreamSynchronize(c->st));
+    double J = 0.0;
+    for (double v : h) J += v;
+    return J;
+}
+
+void zero_planes(cdns_ctx *c, zd *const *pl, int n, size_t count) {
+    for (int k = 0; k < n; ++k) CDNS_HIP(hipMemsetAsync(pl[k], 0, count * sizeof(zd), c->st));
 }
 
 // ---------------------------------------------------- layout conversion --
@@ -652,6 +859,10 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         C->PB = C->alloc<zd>(g.S);
         for (int k = 0; k < 3; ++k) C->VB[k] = C->alloc<zd>(g.S);
         C->SIG0 = C->alloc<zd>((size_t)g.NQ * 2);
+        C->compact = cfg->a_V[0] == cfg->a_V[2] && cfg->b_V[0] == cfg->b_V[2];
+        if (const char *e = getenv("CDNS_SOLVER_TABLES")) C->compact = C->compact && std::string(e) != "complex";
+        if (C->compact)
+            for (int i = 0; i < 9; ++i) C->ctab[i] = C->alloc<double>(g.S);
         C->u_unit = C->alloc<double>(g.JS);
         C->w_unit = C->alloc<double>(g.JS);
         C->q_unit = C->alloc<double>(2);
@@ -1025,6 +1236,172 @@ int cdns_profile_reset(cdns_ctx *c) {
         std::fill(c->kcnt.begin(), c->kcnt.end(), 0LL);
         return 0;
     } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// ------------------------------------------ discrete adjoint (no reference counterpart) --
+int cdns_set_body_force(cdns_ctx *c, const double *force, int where) {
+    try {
+        const Geometry &g = c->g;
+        if (!force) {
+            c->body_on = false;
+            return 0;
+        }
+        if (!c->body[0])
+            for (int k = 0; k < 3; ++k) c->body[k] = c->alloc<zd>(g.S);
+        ref_to_planes(c, force, 3, 3, g.nj, c->body, where, g.JS);
+        c->body_on = true;
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_objective(cdns_ctx *c, int which, const double *velocity, double *grad, double *J, int where) {
+    try {
+        const Geometry &g = c->g;
+        ensure_adjoint(c);
+        zd *u[3] = {c->scr[0], c->scr[1], c->scr[2]}, *gp[3] = {c->scr[3], c->scr[4], c->scr[5]};
+        ref_to_planes(c, velocity, 3, 3, g.nj, u, where, g.JS);
+        const double v = objective(c, which, u, gp);
+        if (J) *J = v;
+        if (grad) {
+            const zd *cg[3] = {gp[0], gp[1], gp[2]};
+            planes_to_ref(c, cg, 3, g.nj, grad, 3, where, g.JS);
+        }
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_adjoint_step(cdns_ctx *c, int it, const double *velocity, double *adj_velocity, double *adj_velocity_prev,
+                      double *adj_nonlinear_prev, double *adj_pressure_bc_prev, double *adj_force, int where) {
+    try {
+        const Geometry &g = c->g;
+        ensure_adjoint(c);
+        if (it < 1) throw std::invalid_argument("iteration numbers start at 1");
+        // the step's synthesised velocity and curl, as the forward step computes them
+        zd *u[3] = {c->scr[0], c->scr[1], c->scr[2]};
+        ref_to_planes(c, velocity, 3, 3, g.nj, u, where, g.JS);
+        curl_planes(c, u, c->om, nullptr);
+        const zd *f6[6] = {u[0], u[1], u[2], c->om[0], c->om[1], c->om[2]};
+        backward_yz(c, f6, 6);
+        ref_to_planes(c, adj_velocity, 3, 3, g.nj, c->a_lu, where, g.JS);
+        ref_to_planes(c, adj_velocity_prev, 3, 3, g.nj, c->a_lup, where, g.JS);
+        ref_to_planes(c, adj_nonlinear_prev, 3, 3, g.nj, c->a_lnp, where, g.JS);
+        ref_to_planes(c, adj_pressure_bc_prev, 1, 1, 2, &c->a_lbp, where, 2);
+        if (adj_force) ref_to_planes(c, adj_force, 3, 3, g.nj, c->a_lf, where, g.JS);
+        adjoint_step(c, it, c->Zbuf, adj_force != nullptr);
+        const zd *o1[3] = {c->a_lu[0], c->a_lu[1], c->a_lu[2]};
+        const zd *o2[3] = {c->a_lup[0], c->a_lup[1], c->a_lup[2]};
+        const zd *o3[3] = {c->a_lnp[0], c->a_lnp[1], c->a_lnp[2]};
+        const zd *o4[1] = {c->a_lbp};
+        planes_to_ref(c, o1, 3, g.nj, adj_velocity, 3, where, g.JS);
+        planes_to_ref(c, o2, 3, g.nj, adj_velocity_prev, 3, where, g.JS);
+        planes_to_ref(c, o3, 3, g.nj, adj_nonlinear_prev, 3, where, g.JS);
+        planes_to_ref(c, o4, 1, 2, adj_pressure_bc_prev, 1, where, 2);
+        if (adj_force) {
+            const zd *o5[3] = {c->a_lf[0], c->a_lf[1], c->a_lf[2]};
+            planes_to_ref(c, o5, 3, g.nj, adj_force, 3, where, g.JS);
+        }
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_adjoint_run(cdns_ctx *c, const double *u0, const double *force, int nsteps, int k_ckpt, int objective_id,
+                     double *grad_u0, double *grad_force, double *J, int where) {
+    void *ck = nullptr, *seg = nullptr;
+    void *zkeep = c ? c->Zbuf : nullptr;
+    try {
+        const Geometry &g = c->g;
+        if (nsteps < 1 || k_ckpt < 1) throw std::invalid_argument("nsteps and the checkpoint interval must be >= 1");
+        ensure_adjoint(c);
+        if (force) {
+            if (cdns_set_body_force(c, force, where) != 0) throw std::runtime_error(c->err);
+        } else {
+            c->body_on = false;
+        }
+        // initial state: u0, no history (steps 1 and 2 weight it by zero)
+        ref_to_planes(c, u0, 3, 3, g.nj, c->u, where, g.JS);
+        zero_planes(c, c->up, 3, g.S);
+        zero_planes(c, c->nlp, 3, g.S);
+        CDNS_HIP(hipMemsetAsync(c->bc_prev, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
+        const int kc = std::min(k_ckpt, nsteps);
+        const int nck = (nsteps + kc - 1) / kc;
+        // checkpoint: u, u_prev, N_prev (9 planes) and B_prev
+        const size_t ckn = (size_t)9 * g.S + (size_t)g.NQ * 2;
+        CDNS_HIP(hipMalloc(&ck, (size_t)nck * ckn * sizeof(zd)));
+        CDNS_HIP(hipMalloc(&seg, (size_t)kc * 6 * g.ZS * c->esz));
+        auto save = [&](int k) {
+            zd *base = reinterpret_cast<zd *>(ck) + (size_t)k * ckn;
+            for (int i = 0; i < 3; ++i) {
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)i * g.S, c->u[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)(3 + i) * g.S, c->up[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)(6 + i) * g.S, c->nlp[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+            }
+            CDNS_HIP(hipMemcpyAsync(base + (size_t)9 * g.S, c->bc_prev, (size_t)g.NQ * 2 * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+        };
+        auto restore = [&](int k) {
+            const zd *base = reinterpret_cast<const zd *>(ck) + (size_t)k * ckn;
+            for (int i = 0; i < 3; ++i) {
+                CDNS_HIP(hipMemcpyAsync(c->u[i], base + (size_t)i * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(c->up[i], base + (size_t)(3 + i) * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(c->nlp[i], base + (size_t)(6 + i) * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+            }
+            CDNS_HIP(hipMemcpyAsync(c->bc_prev, base + (size_t)9 * g.S, (size_t)g.NQ * 2 * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
+        };
+        auto fwd = [&](int it) {
+            if (it >= 3) ensure_tables(c, g_for_iteration(c, it));
+            step(c, it);
+        };
+        for (int it = 1; it <= nsteps; ++it) {
+            if ((it - 1) % kc == 0) save((it - 1) / kc);
+            fwd(it);
+        }
+        // terminal objective and its gradient start the adjoint
+        const double Jv = objective(c, objective_id, c->u, c->a_lu);
+        if (J) *J = Jv;
+        zero_planes(c, c->a_lup, 3, g.S);
+        zero_planes(c, c->a_lnp, 3, g.S);
+        zero_planes(c, c->a_lf, 3, g.S);
+        CDNS_HIP(hipMemsetAsync(c->a_lbp, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
+        for (int k = nck - 1; k >= 0; --k) {
+            const int it0 = k * kc + 1, it1 = std::min((k + 1) * kc, nsteps);
+            restore(k);
+            // recompute the segment, keeping each step's synthesised (u, curl u)
+            for (int it = it0; it <= it1; ++it) {
+                c->Zbuf = (char *)seg + (size_t)(it - it0) * 6 * g.ZS * c->esz;
+                fwd(it);
+            }
+            c->Zbuf = zkeep;
+            for (int it = it1; it >= it0; --it)
+                adjoint_step(c, it, (const char *)seg + (size_t)(it - it0) * 6 * g.ZS * c->esz, force != nullptr);
+        }
+        c->Zbuf = zkeep;
+        // leave the tables of the steady iterations in place for later forward steps
+        if (nsteps >= 2) ensure_tables(c, g_for_iteration(c, 2));
+        const zd *o1[3] = {c->a_lu[0], c->a_lu[1], c->a_lu[2]};
+        planes_to_ref(c, o1, 3, g.nj, grad_u0, 3, where, g.JS);
+        if (grad_force) {
+            const zd *o2[3] = {c->a_lf[0], c->a_lf[1], c->a_lf[2]};
+            planes_to_ref(c, o2, 3, g.nj, grad_force, 3, where, g.JS);
+        }
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        CDNS_HIP(hipFree(ck));
+        CDNS_HIP(hipFree(seg));
+        return 0;
+    } catch (const std::exception &e) {
+        if (c) c->Zbuf = zkeep;
+        if (c && c->st) (void)hipStreamSynchronize(c->st);
+        if (ck) (void)hipFree(ck);
+        if (seg) (void)hipFree(seg);
         return fail(c, e);
     }
 }

@@ -9,6 +9,7 @@ namespace cdns {
 struct YArgs {
     const zd *spec[6]; // ybwd inputs: spectral component planes
     zd *spec_out[6];   // yfwd outputs
+    const zd *add[6];  // yfwd: planes added to the outputs (a body force), or null
     void *y;           // Y fields, field f at y + f*YS (element type by precision)
     const double *coef, *coef2;
     int nj, nje, njo, JS, NQ, NLH, NY2, nwj, ng;
@@ -38,7 +39,9 @@ struct XArgs {
     int nkx, NL, NZ2, P;
     int rf;
     double *phys; // reference-layout physical field for the single-field stages
-    int fmap[6];  // product: field indices (at in + fmap[i]*in_fs) of a0 a1 a2 b0 b1 b2; out = a x b
+    // product a x b: input fields a0 a1 a2 b0 b1 b2 and the three output fields
+    const void *fin[6];
+    void *fout[3];
 };
 
 void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
@@ -109,6 +112,10 @@ struct SolveArgs {
     // tables
     const double *ad_p, *ad_v;
     const zd *PI[2], *VI[2][3], *ID[2], *PB, *VB[3], *SIG0;
+    // compact form of the influence and B responses (used when u and w share their wall
+    // conditions): P and v are real, u = i kx phi, w = i kz phi with phi real
+    int compact;
+    const double *PIr[2], *VIv[2], *VIf[2], *PBr, *VBv, *VBf;
     const double *u_unit, *w_unit;
     const double *q_unit;
     double grad_x, grad_z, qx_mean, qz_mean;
@@ -119,7 +126,37 @@ struct SolveArgs {
     zd *sig_out;  // b problem: sigma0 [q][2]
     double *flow; // step: dpdx, dpdz
 };
+// Discrete adjoint of the step (no reference counterpart): the transposed solve with the RHS,
+// history and wall-condition transposes (k_adj_solve), the output weights and curl^T of the
+// transposed nonlinear term (k_adj_combine), and the terminal objective (k_objective).
+struct AdjArgs {
+    int ny, JS, NQ, nzk, nkx, NX2, NY2, collide, which;
+    const double *WAVX, *WAVZ, *WL, *WD, *WR;
+    double Re, G_pre, g_v, pa, pb, XL, ZL;
+    double aV[3], bV[3];
+    double a0, a1, b0, b1;
+    int grad_pre;
+    const double *ad_p, *ad_v;
+    const zd *PI[2], *VI[2][3], *ID[2], *VB[3], *SIG0;
+    const double *u_unit, *w_unit, *q_unit;
+    // adjoint state (in/out): velocity, previous velocity, previous nonlinear term, previous
+    // pressure wall condition [q][2]; lf accumulates the body-force gradient (or null)
+    zd *lu[3], *lup[3], *lnp[3], *lbp, *lf[3];
+    zd *lNs[3]; // out: weighted adjoint of the nonlinear term (input of the transposed synthesis)
+    zd *lw[3];  // out: adjoint of the curl from the wall condition
+    const zd *lUs[3], *lWs[3]; // transposed-product results (inputs of k_adj_combine)
+    zd *grad[3];
+    double *Jpart;
+};
+void launch_adj_solve(const AdjArgs &a, int E, hipStream_t st);
+void launch_adj_combine(const AdjArgs &a, int E, hipStream_t st);
+void launch_adj_fold(const AdjArgs &a, hipStream_t st);
+void launch_objective(const AdjArgs &a, int E, hipStream_t st);
+
 void launch_solve(const SolveArgs &a, int E, hipStream_t st);
+// real tables of the compact form from the complex setup tables
+void launch_compact_tables(const SolveArgs &a, double *PIr0, double *PIr1, double *VIv0, double *VIv1,
+                           double *VIf0, double *VIf1, double *PBr, double *VBv, double *VBf, hipStream_t st);
 void launch_build_ad(const double *WAVX, const double *WAVZ, const double *WL, const double *WD,
                      const double *WR, double g, int ny, int JS, int NQ, int nzk, double *ad,
                      double *wn, hipStream_t st);

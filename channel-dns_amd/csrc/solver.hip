@@ -298,7 +298,18 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
 
     // ---- influence-matrix and sigma corrections (eqsolve_pv.cpp:407-445)
     zd c0 = {0.0, 0.0}, c1 = {0.0, 0.0}, sig[2] = {{0.0, 0.0}, {0.0, 0.0}};
-    if (a.correct) {
+    if (a.correct && a.compact && mode == MODE_STEP) {
+        const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
+        c0 = cmul(div1, I1[0]) - cmul(div0, I1[1]);
+        c1 = cmul(div0, I0[1]) - cmul(div1, I0[0]);
+        if (a.sigma) {
+            const zd ptop = col_at<CH>(pc, ny) + (a.PIr[0][off + ny] * c0 + a.PIr[1][off + ny] * c1);
+            const zd vt0 = col_at<CH>(vc[1], ny) + (a.VIv[0][off + ny] * c0 + a.VIv[1][off + ny] * c1);
+            const zd vt1 = col_at<CH>(vc[1], ny - 1) + (a.VIv[0][off + ny - 1] * c0 + a.VIv[1][off + ny - 1] * c1);
+            sig[0] = cdiv((-wn_v) * vt0 - a.Re * tail0, a.SIG0[(size_t)q * 2 + 0]);
+            sig[1] = cdiv((-wn_v) * vt1 + a.Re * ((2.0 * ny) * ptop - tail1), a.SIG0[(size_t)q * 2 + 1]);
+        }
+    } else if (a.correct) {
         const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
         c0 = cmul(div1, I1[0]) - cmul(div0, I1[1]);
         c1 = cmul(div0, I0[1]) - cmul(div1, I0[0]);
@@ -312,6 +323,38 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
         }
     }
     zd P[2][CH], V[3][2][CH];
+    if (a.correct && a.sigma && a.compact && mode == MODE_STEP) {
+        // P += c0 P0 + c1 P1 + s_(1-p) PB;  v += c0 v0 + c1 v1 + s_p vB;
+        // (u, w) += i (kx, kz) (c0 f0 + c1 f1 + s_(1-p) fB)
+        double r0[2][CH], r1[2][CH], rb[2][CH];
+        load_rcol<CH>(a.PIr[0] + off, lane, ny, r0);
+        load_rcol<CH>(a.PIr[1] + off, lane, ny, r1);
+        load_rcol<CH>(a.PBr + off, lane, ny, rb);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+                P[p][c] = pc[p][c] + ((r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[1 - p]);
+        load_rcol<CH>(a.VIv[0] + off, lane, ny, r0);
+        load_rcol<CH>(a.VIv[1] + off, lane, ny, r1);
+        load_rcol<CH>(a.VBv + off, lane, ny, rb);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+                V[1][p][c] = vc[1][p][c] + ((r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[p]);
+        load_rcol<CH>(a.VIf[0] + off, lane, ny, r0);
+        load_rcol<CH>(a.VIf[1] + off, lane, ny, r1);
+        load_rcol<CH>(a.VBf + off, lane, ny, rb);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const zd phi = (r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[1 - p];
+                V[0][p][c] = vc[0][p][c] + ikmul(kx, phi);
+                V[2][p][c] = vc[2][p][c] + ikmul(kz, phi);
+            }
+    } else {
     {
         zd i0[2][CH], i1[2][CH], bb[2][CH];
         if (a.correct) {
@@ -346,6 +389,7 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
                 if (a.sigma) v = v + cmul(sig[k == 1 ? p : 1 - p], bb[p][c]);
                 V[k][p][c] = v;
             }
+    }
     }
 
     if (mode == MODE_STEP && zero) {
@@ -409,6 +453,38 @@ void launch_solve(const SolveArgs &a, int E, hipStream_t st) {
     if (E == 2) hipLaunchKernelGGL((k_solve<1>), grid, block, 0, st, a);
     else if (E == 4) hipLaunchKernelGGL((k_solve<2>), grid, block, 0, st, a);
     else throw std::runtime_error("unsupported column length");
+}
+
+// Compact tables: the influence and B responses come from real wall data and real operators,
+// so P and v are real and u, w are i kx phi, i kz phi with one real phi when u and w share their
+// wall conditions (eqsolve_pv.cpp:266-303, 452-477).
+__global__ void k_compact_tables(SolveArgs a, double *PIr0, double *PIr1, double *VIv0, double *VIv1, double *VIf0,
+                                 double *VIf1, double *PBr, double *VBv, double *VBf) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)a.NQ * a.JS) return;
+    const int q = (int)(e / a.JS);
+    const int kxi = q / a.nzk, kzi = q - kxi * a.nzk;
+    const double kx = a.WAVX[kxi], kz = a.WAVZ[kzi];
+    auto phi = [&](const zd *u, const zd *w) {
+        if (kx != 0.0) return u[e].y / kx;
+        if (kz != 0.0) return w[e].y / kz;
+        return 0.0;
+    };
+    PIr0[e] = a.PI[0][e].x;
+    PIr1[e] = a.PI[1][e].x;
+    VIv0[e] = a.VI[0][1][e].x;
+    VIv1[e] = a.VI[1][1][e].x;
+    VIf0[e] = phi(a.VI[0][0], a.VI[0][2]);
+    VIf1[e] = phi(a.VI[1][0], a.VI[1][2]);
+    PBr[e] = a.PB[e].x;
+    VBv[e] = a.VB[1][e].x;
+    VBf[e] = phi(a.VB[0], a.VB[2]);
+}
+void launch_compact_tables(const SolveArgs &a, double *PIr0, double *PIr1, double *VIv0, double *VIv1,
+                           double *VIf0, double *VIf1, double *PBr, double *VBv, double *VBf, hipStream_t st) {
+    const size_t n = (size_t)a.NQ * a.JS;
+    hipLaunchKernelGGL(k_compact_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, PIr0, PIr1, VIv0,
+                       VIv1, VIf0, VIf1, PBr, VBv, VBf);
 }
 
 // build_wn_ad (eqsolve_pv.cpp:148-168): continued-fraction recurrence, one thread per pair.

@@ -151,10 +151,15 @@ __global__ void __launch_bounds__(512) k_yfwd(YArgs a) {
     }
     if (j >= a.nj) return;
     zd *out = a.spec_out[f];
+    const zd *add = a.add[f];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int q = q0 + g * C + c;
-        if (q < a.NQ) out[(size_t)q * a.JS + j] = {double(acc[c].x), double(acc[c].y)};
+        if (q < a.NQ) {
+            zd v = {double(acc[c].x), double(acc[c].y)};
+            if (add) v = v + add[(size_t)q * a.JS + j];
+            out[(size_t)q * a.JS + j] = v;
+        }
     }
 }
 
@@ -248,7 +253,7 @@ __device__ inline cpx<R> pack_c2r_single(const cpx<R> *A, int k, int N, int K) {
     return a;
 }
 
-// One block: P points (l, iz); input fields fmap[0..5] = a (3 components), b (3 components).
+// One block: P points (l, iz); input fields fin[0..5] = a (3 components), b (3 components).
 template <typename R, int IT>
 __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
     extern __shared__ unsigned char smem_raw[];
@@ -256,7 +261,6 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
     const int P = a.P, N = a.plan.n, K = a.nkx - 1;
     const int np = a.NL * a.NZ2;
     const int p0 = blockIdx.x * P;
-    const cpx<R> *__restrict__ Z = reinterpret_cast<const cpx<R> *>(a.in);
     // 1. pack (u,v), (w,wx), (wy,wz) per point
     for (int e = threadIdx.x; e < 3 * P * N; e += blockDim.x) {
         const int line = e / N, k = e - line * N;
@@ -264,8 +268,8 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
         const int pt = p0 + pp;
         cpx<R> v = {R(0), R(0)};
         if (pt < np) {
-            const cpx<R> *A = Z + (size_t)a.fmap[2 * gsel] * a.in_fs + (size_t)pt * a.nkx;
-            const cpx<R> *B = Z + (size_t)a.fmap[2 * gsel + 1] * a.in_fs + (size_t)pt * a.nkx;
+            const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * gsel]) + (size_t)pt * a.nkx;
+            const cpx<R> *B = reinterpret_cast<const cpx<R> *>(a.fin[2 * gsel + 1]) + (size_t)pt * a.nkx;
             v = pack_c2r(A, B, k, N, K);
         }
         buf[e] = v;
@@ -312,7 +316,6 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
     fft_block<R, IT>(buf, 3 * (P / 2), a.plan, reinterpret_cast<const cpx<R> *>(a.tw), -1);
     // 3. unpack two real spectra per line, x 1/NX2, keep kx <= NXS
     const R half = R(0.5) / R(N);
-    cpx<R> *ZN = reinterpret_cast<cpx<R> *>(a.out);
     for (int e = threadIdx.x; e < 3 * (P / 2) * (K + 1); e += blockDim.x) {
         const int line = e / (K + 1), k = e - line * (K + 1);
         const int h = line / 3, sel = line - h * 3;
@@ -324,8 +327,8 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
         if (sel < 2) { pa = pb = 2 * h + sel; ca = 0; cb = 1; }
         else { pa = 2 * h; pb = 2 * h + 1; ca = cb = 2; }
         const int pta = p0 + pa, ptb = p0 + pb;
-        if (pta < np) ZN[(size_t)ca * a.out_fs + (size_t)pta * a.nkx + k] = first;
-        if (ptb < np) ZN[(size_t)cb * a.out_fs + (size_t)ptb * a.nkx + k] = second;
+        if (pta < np) reinterpret_cast<cpx<R> *>(a.fout[ca])[(size_t)pta * a.nkx + k] = first;
+        if (ptb < np) reinterpret_cast<cpx<R> *>(a.fout[cb])[(size_t)ptb * a.nkx + k] = second;
     }
 }
 

@@ -57,14 +57,13 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
     const int np = a.NL * a.NZ2, K = a.nkx - 1;
     const int p = blockIdx.x * NG + g;
     const bool valid = p < np;
-    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in);
     cpx<R> *B = buf[g];
     cpx<R> ph[3][MX];
     sfor<0, 3>([&](auto line) {
         constexpr int LN = decltype(line)::value;
         cpx<R> x[MX];
-        const cpx<R> *A = Z + (size_t)a.fmap[2 * LN] * a.in_fs + (size_t)p * a.nkx;
-        const cpx<R> *Bv = Z + (size_t)a.fmap[2 * LN + 1] * a.in_fs + (size_t)p * a.nkx;
+        const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]) + (size_t)p * a.nkx;
+        const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]) + (size_t)p * a.nkx;
         sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
         if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = herm_pack(A, Bv, t + N1 * decltype(n2)::value, N, K); });
         two_pass<R, N1, N2, +1>(x, t, tw, B, wsync);
@@ -85,7 +84,8 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         });
     }
     const R half = R(0.5) / R(N);
-    cpx<R> *ZN = reinterpret_cast<cpx<R> *>(a.out);
+    cpx<R> *Nx = reinterpret_cast<cpx<R> *>(a.fout[0]), *Ny = reinterpret_cast<cpx<R> *>(a.fout[1]),
+           *Nz = reinterpret_cast<cpx<R> *>(a.fout[2]);
     // forward of (Nx + i Ny): transposed plan, input already in place
     two_pass<R, N2, N1, -1>(F, t, tw, B, wsync);
     if (t < N1) sfor<0, N2>([&](auto k1) { B[t + N1 * decltype(k1)::value] = F[k1]; });
@@ -94,8 +94,8 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         for (int k = t; k <= K; k += G) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm, d = f - gm;
-            ZN[(size_t)0 * a.out_fs + (size_t)p * a.nkx + k] = {half * s.x, half * s.y};
-            ZN[(size_t)1 * a.out_fs + (size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
+            Nx[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            Ny[(size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
         }
     wsync();
     two_pass<R, N2, N1, -1>(Fz, t, tw, B, wsync);
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         for (int k = t; k <= K; k += G) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm;
-            ZN[(size_t)2 * a.out_fs + (size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            Nz[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
         }
 }
 
@@ -189,18 +189,38 @@ __global__ void __launch_bounds__(64) k_xfwd_phys_rf(XArgs a) {
 }
 
 // ------------------------------------------------------------------ z stages --
-// backward: one line per group = (f, l, kx); slot k = t + N1*n2 of the kz axis.
-template <typename R, int N1, int N2>
-__global__ void __launch_bounds__(64) k_zbwd_rf(ZArgs a) {
+// A block owns KB (16, or 8 for the largest lengths) consecutive kx lines of one (field, l), one
+// lane group per line.  The kx-contiguous side (Z[l][iz][kx]) goes through an LDS tile [iz][kx]
+// so that HBM sees whole KB-line rows (256 B in FP64) instead of one 16-byte element per line.
+template <typename R, int N1, int N2> struct ZTile {
     using Gm = Geo<N1, N2>;
-    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    static constexpr int N = Gm::N;
+    static constexpr int lds_of(int kb) {
+        return (kb * Gm::LB > N * (kb + 1) ? kb * Gm::LB : N * (kb + 1)) + N; // tile + roots
+    }
+    // 16 lines per block when the tile fits 64 KB of LDS, else 8
+    static constexpr int KB = lds_of(16) * (int)sizeof(cpx<R>) <= 65536 ? 16 : 8;
+    static constexpr int WAVES = KB / Gm::NG, PITCH = KB + 1;
+    static constexpr int LDS = lds_of(KB) - N; // complex entries of the tile
+};
+template <typename R, int N1, int N2> constexpr int kZThreads = 64 * ZTile<R, N1, N2>::WAVES;
+
+// backward: slot s of the kz axis <- retained kz, FFT over the slots, Z[l][iz][kx]
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
+    using Gm = Geo<N1, N2>;
+    using T = ZTile<R, N1, N2>;
+    constexpr int kZLines = T::KB;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB, PITCH = T::PITCH;
     __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> buf[NG][LB];
-    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    __shared__ cpx<R> lds[T::LDS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
+    const int line = wv * NG + g;
     load_roots<R, N>(tw, a.tw);
     __syncthreads();
-    const int kx = blockIdx.x * NG + g, l = blockIdx.y, f = blockIdx.z;
-    const bool valid = kx < a.nkx;
+    const int kx0 = blockIdx.x * kZLines, kx = kx0 + line, l = blockIdx.y, f = blockIdx.z;
+    const int nl = min(kZLines, a.nkx - kx0);
+    const bool valid = line < nl;
     const int NZS = a.NZS;
     const cpx<R> *Y = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * a.NQ +
                       (size_t)(valid ? kx : 0) * a.nzk;
@@ -218,32 +238,44 @@ __global__ void __launch_bounds__(64) k_zbwd_rf(ZArgs a) {
             if (kz != (1 << 30)) x[n2] = Y[kz + NZS];
         });
     }
-    two_pass<R, N1, N2, +1>(x, t, tw, buf[g], wsync);
-    if (valid && t < N2) {
-        cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx;
-        sfor<0, N1>([&](auto k1) { Z[(size_t)(t + N2 * decltype(k1)::value) * a.nkx] = x[k1]; });
+    two_pass<R, N1, N2, +1>(x, t, tw, lds + line * LB, wsync);
+    // two_pass ends with a barrier: the tile may now overwrite the transpose buffers
+    if (valid && t < N2) sfor<0, N1>([&](auto k1) { lds[(t + N2 * decltype(k1)::value) * PITCH + line] = x[k1]; });
+    __syncthreads();
+    cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx0;
+    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
+        const int iz = e / nl, li = e - iz * nl;
+        Z[(size_t)iz * a.nkx + li] = lds[iz * PITCH + li];
     }
 }
 
-// forward: line (f, l, kx) over iz; keep the retained kz, x 1/NZ2.
+// forward: Z[l][iz][kx] tile -> FFT over iz per kx line, keep the retained kz, x 1/NZ2
 template <typename R, int N1, int N2>
-__global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
+__global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_rf(ZArgs a) {
     using Gm = Geo<N1, N2>;
-    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    using T = ZTile<R, N1, N2>;
+    constexpr int kZLines = T::KB;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB, PITCH = T::PITCH;
     __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> buf[NG][LB];
-    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    __shared__ cpx<R> lds[T::LDS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
+    const int line = wv * NG + g;
     load_roots<R, N>(tw, a.tw);
-    __syncthreads();
-    const int kx = blockIdx.x * NG + g, l = blockIdx.y, f = blockIdx.z;
-    const bool valid = kx < a.nkx;
+    const int kx0 = blockIdx.x * kZLines, kx = kx0 + line, l = blockIdx.y, f = blockIdx.z;
+    const int nl = min(kZLines, a.nkx - kx0);
+    const bool valid = line < nl;
     const int NZS = a.NZS;
-    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx +
-                      (valid ? kx : 0);
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx + kx0;
+    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
+        const int iz = e / nl, li = e - iz * nl;
+        lds[iz * PITCH + li] = Z[(size_t)iz * a.nkx + li];
+    }
+    __syncthreads();
     cpx<R> x[MX];
     sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
-    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[(size_t)(t + N1 * decltype(n2)::value) * a.nkx]; });
-    two_pass<R, N1, N2, -1>(x, t, tw, buf[g], wsync);
+    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = lds[(t + N1 * decltype(n2)::value) * PITCH + line]; });
+    __syncthreads(); // the transpose buffers below reuse the tile
+    two_pass<R, N1, N2, -1>(x, t, tw, lds + line * LB, wsync);
     if (valid && t < N2) {
         const R inv = R(1.0 / N);
         cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
@@ -272,12 +304,14 @@ template <typename R, int N1, int N2> static void go_xfwd(const XArgs &a, hipStr
                        st, a);
 }
 template <typename R, int N1, int N2> static void go_zbwd(const ZArgs &a, int nf, hipStream_t st) {
-    using Gm = Geo<N1, N2>;
-    hipLaunchKernelGGL((k_zbwd_rf<R, N1, N2>), dim3((a.nkx + Gm::NG - 1) / Gm::NG, a.NL, nf), dim3(64), 0, st, a);
+    using T = ZTile<R, N1, N2>;
+    hipLaunchKernelGGL((k_zbwd_rf<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES), 0,
+                       st, a);
 }
 template <typename R, int N1, int N2> static void go_zfwd(const ZArgs &a, int nf, hipStream_t st) {
-    using Gm = Geo<N1, N2>;
-    hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3((a.nkx + Gm::NG - 1) / Gm::NG, a.NL, nf), dim3(64), 0, st, a);
+    using T = ZTile<R, N1, N2>;
+    hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES), 0,
+                       st, a);
 }
 
 // Supported lengths: N = N1 * N2 with N1, N2 <= 16.

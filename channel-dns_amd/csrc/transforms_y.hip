@@ -251,15 +251,22 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
                 ym = {pm[0], pm[1]};
             }
             zd *out = a.spec_out[f] + (size_t)q * a.JS;
+            const zd *add = a.add[f] ? a.add[f] + (size_t)q * a.JS : nullptr;
 #pragma unroll
             for (int t = 0; t < JTP; ++t) {
                 const int m = m0 + 16 * t + col;
                 const R fm = Fm[16 * t + col];
                 if (m < nje) {
                     const R re = accE[c][t][2 * h] + fm * ym.x, im = accE[c][t][2 * h + 1] + fm * ym.y;
-                    out[2 * m] = {double(re), double(im)};
+                    zd v = {double(re), double(im)};
+                    if (add) v = v + add[2 * m];
+                    out[2 * m] = v;
                 }
-                if (m < njo) out[2 * m + 1] = {double(accO[c][t][2 * h]), double(accO[c][t][2 * h + 1])};
+                if (m < njo) {
+                    zd v = {double(accO[c][t][2 * h]), double(accO[c][t][2 * h + 1])};
+                    if (add) v = v + add[2 * m + 1];
+                    out[2 * m + 1] = v;
+                }
             }
         }
     }

@@ -101,6 +101,32 @@ int cdns_profile(cdns_ctx *ctx, int enable);
 int cdns_profile_read(cdns_ctx *ctx, char *names, long long *counts, double *total_ms, int cap);
 int cdns_profile_reset(cdns_ctx *ctx);
 
+/* ---- Discrete adjoint of the step (no reference counterpart: the reference integrates
+ * forward only).  FP64 contexts.  Adjoint arrays use the layouts of the state they belong to;
+ * gradients are d/d(re) + i d/d(im) of a real objective. */
+
+/* Constant body force f (spectral vector, reference layout) added to every step's nonlinear
+ * term, so that it enters the RHS with weight b0 + b1 = 1 and the pressure wall condition
+ * through f_y; NULL removes it. */
+int cdns_set_body_force(cdns_ctx *ctx, const double *force, int where);
+/* Terminal objective of a velocity: which = 0 kinetic energy 1/2 int |u|^2 dV, 1 mean wall
+ * shear 1/2 (du/dy(-1) - du/dy(+1)) of the plane mean.  Writes J and (if grad) dJ/du. */
+int cdns_objective(cdns_ctx *ctx, int which, const double *velocity, double *grad, double *J,
+                   int where);
+/* Transpose of step `it` at the state whose velocity is `velocity`: the adjoints of the state
+ * after the step (velocity, velocity_prev, nonlinear_prev: vectors; pressure_bc_prev: boundary)
+ * are replaced by those of the state before it; adj_force (or NULL) accumulates the body-force
+ * gradient.  The solver tables are set up for iteration it if they are not. */
+int cdns_adjoint_step(cdns_ctx *ctx, int it, const double *velocity, double *adj_velocity,
+                      double *adj_velocity_prev, double *adj_nonlinear_prev,
+                      double *adj_pressure_bc_prev, double *adj_force, int where);
+/* Gradient of objective(u(T)) after iterations 1..nsteps from u0 (zero history) with body force
+ * `force` (or none): forward with a checkpoint every k_ckpt steps, then per segment a forward
+ * recomputation that keeps each step's synthesised (u, curl u), and the adjoint steps in reverse.
+ * Writes J, grad_u0 and (if non-NULL) grad_force.  The context's state is overwritten. */
+int cdns_adjoint_run(cdns_ctx *ctx, const double *u0, const double *force, int nsteps, int k_ckpt,
+                     int objective, double *grad_u0, double *grad_force, double *J, int where);
+
 #ifdef __cplusplus
 }
 #endif
