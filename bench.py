@@ -41,8 +41,57 @@ def algorithmic_bytes(cfg, esz):
         "x_product": 6 * Z * esz + 3 * Z * esz,
         "z_forward": 3 * Z * esz + 3 * Y * esz,
         "y_forward": 3 * Y * esz + 3 * S * zc,
-        # u, u_prev, N, N_prev, 12 influence/B columns, 2 real ad columns in; u, P out
-        "solve_step": (12 * S + 12 * S) * zc + 2 * S * 8 + 4 * S * zc,
+        # u, u_prev, N, N_prev in; the influence/B responses as 9 real columns (P and v real,
+        # u = i kx phi, w = i kz phi) and 2 real ad columns; u, P out
+        "solve_step": 12 * S * zc + 9 * S * 8 + 2 * S * 8 + 4 * S * zc,
+    }
+
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload
+# (FP64), each scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0); profiles/step_retau180_hbm_traffic.md holds the runs.
+MEASURED_TRAFFIC = {
+    "x_product": 365.4e6,
+}
+
+
+def adjoint_timing(steps=200, k_ckpt=20, dt=5e-4):
+    """Forward and checkpointed adjoint (gradient of the terminal kinetic energy w.r.t. the
+    initial velocity and a body force) on the 64x65x64 grid (product 96x97x96)."""
+    import numpy as np
+    import torch
+
+    import cdns
+    from cdns import config, fields
+
+    cfg = dict(config.channel_64(), DT=dt)
+    u0 = fields.seeded_velocity(cfg, seed=64, mean="log", spectrum="k53", amplitude=0.1)
+    f = 0.01 * fields.seeded_velocity(cfg, seed=65, mean="laminar", kmax=4, amplitude=1.0)
+    f[cfg["NZS"], :, 0, :] = 0
+    dev = cdns.Channel(cfg)
+    dev.set_body_force(f)
+    dev.set_state("velocity", u0)
+    dev.run(2, it_begin=1)
+    dev.run(steps, it_begin=3)  # warm-up of the same shape
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.run(steps, it_begin=3)
+    dev.sync()
+    t_fwd = time.perf_counter() - t0
+    dev.adjoint_run(u0, f, steps, k_ckpt, 0)  # warm-up
+    t0 = time.perf_counter()
+    J, gu, gf = dev.adjoint_run(u0, f, steps, k_ckpt, 0)
+    t_adj = time.perf_counter() - t0
+    dev.close()
+    return {
+        "workload": f"64x65x64 (product 96x97x96), dt={dt}, {steps} steps, checkpoint every {k_ckpt}, "
+                    "d(KE(T))/d(u0, body force)",
+        "forward_ms_per_step": 1e3 * t_fwd / steps,
+        "adjoint_run_ms": 1e3 * t_adj,
+        "adjoint_run_over_forward": t_adj / t_fwd,
+        "backward_pass_over_forward": (t_adj - t_fwd) / t_fwd,
+        "finite": bool(np.isfinite(J) and np.isfinite(gu).all() and np.isfinite(gf).all()),
     }
 
 
@@ -70,6 +119,24 @@ def cpu_baseline(cfg, u0, seconds_target=15.0, threads=None):
             "sample": f"{k} steps (it=3..{2 + k}) of the same workload, oracle on {n} OpenMP threads, {el:.1f} s"}
 
 
+def counter_run(cdns, cfg, u0):
+    """Steps 1-5 of the FP64 step, then a plain device copy of known size (256 MiB each way)
+    whose counter readings scale FETCH_SIZE and WRITE_SIZE to bytes."""
+    import torch
+
+    dev = cdns.Channel(cfg)
+    dev.set_state("velocity", u0)
+    dev.run(5, it_begin=1)
+    dev.sync()
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dev.close()
+    print(json.dumps({"counter_run": "done", "steps": 5, "copy_bytes": x.numel() * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +145,11 @@ def main():
     ap.add_argument("--workload", default="retau180")
     ap.add_argument("--dt", type=float, default=5e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--counters", action="store_true",
+                    help="no timing: run 5 FP64 steps and three 256 MiB device copies (the calibration "
+                         "of FETCH_SIZE / WRITE_SIZE) for a rocprofv3 --pmc pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs one GPU per process; multi-GPU runs are not part of this build",
@@ -94,6 +165,9 @@ def main():
     cfg = dict(config.WORKLOADS[args.workload](), DT=args.dt)
     u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
     warm = max(args.warmup, 2)
+    if args.counters:
+        counter_run(cdns, cfg, u0)
+        return
 
     res = {}
     for prec, name in ((cdns.FP64, "f64"), (cdns.MIXED, "mixed")):
@@ -135,8 +209,9 @@ def main():
     dom = max((k for k in kern if k in algo), key=lambda k: kern[k]["ms_per_step"])
     dk = kern[dom]
     roof = {"kernel": dom, "bound": "hbm", "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": dk["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "frac": dk["achieved_GBs"] / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dom),
             "algorithmic_bytes_per_launch": algo[dom], "ms_per_launch": dk["ms_per_launch"]}
+    adj = None if args.no_adjoint else adjoint_timing()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, u0, args.cpu_seconds)
@@ -165,7 +240,8 @@ def main():
                   "speedup_vs_f64": mixed_sps / steps_per_s, "finite": res["mixed"]["finite"],
                   "error_vs_f64_after_steps": warm + args.steps, "error_vs_f64": mixed_err,
                   "kernels_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in res["mixed"]["prof"].items()}},
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": None if adj is None else adj["backward_pass_over_forward"],
+        "adjoint": adj,
         "kernels": kern,
         "finite": r["finite"],
     }

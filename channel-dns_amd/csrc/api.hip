@@ -85,6 +85,7 @@ struct cdns_ctx {
     int KT = 13, P = 4;
     int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
     int ymfma = 1; // y stages on the matrix cores (CDNS_YSTAGE=vector selects the vector-ALU kernels)
+    int xpair = 1; // x-stage product FFT rows on lane pairs (CDNS_XPRODUCT=single keeps one lane per row)
     std::vector<double> hWAVX, hWAVZ;
 
     // solver tables
@@ -244,6 +245,7 @@ XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs,
     a.NZ2 = c->g.NZ2;
     a.P = P;
     a.rf = c->rf;
+    a.pair = c->xpair;
     for (int i = 0; i < 6; ++i) a.fin[i] = in ? (const char *)in + i * in_fs * c->esz : nullptr;
     for (int i = 0; i < 3; ++i) a.fout[i] = out ? (char *)out + i * out_fs * c->esz : nullptr;
     return a;
@@ -740,6 +742,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         c->fp32 = cfg->precision == CDNS_MIXED;
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
         if (const char *e = getenv("CDNS_YSTAGE")) c->ymfma = std::string(e) != "vector";
+        if (const char *e = getenv("CDNS_XPRODUCT")) c->xpair = std::string(e) != "single";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;
         g.nzk = 2 * g.NZS + 1;

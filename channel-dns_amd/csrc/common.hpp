@@ -239,4 +239,16 @@ struct HipError : std::runtime_error {
             throw ::cdns::HipError(std::string(#x) + ": " + hipGetErrorString(e_));                  \
     } while (0)
 
+// Workgroups are dealt round-robin to the MI355X's 8 XCDs, each with its own L2.  Blocks that
+// read the same data should run on one XCD at about the same time: for a 1-D grid of nblocks,
+// this returns a logical block id such that each XCD receives one contiguous, in-order range.
+__device__ inline int xcd_block(int nblocks) {
+    constexpr int X = 8;
+    const int b = blockIdx.x;
+    const int per = (nblocks + X - 1) / X;
+    const int tall = nblocks % X == 0 ? X : nblocks % X;
+    const int xcd = b % X, local = b / X;
+    return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
 } // namespace cdns

@@ -16,6 +16,7 @@ struct YArgs {
     size_t YS;
     int mfma;                   // use the matrix-core kernels when the tables fit
     int has_mid, rows_m, njp, KL; // matrix-core shape (set by their launchers)
+    int parts, ncg;               // matrix-core grid: parts per column group, column groups
 };
 
 // z stages (FFT over kz).
@@ -27,6 +28,7 @@ struct ZArgs {
     FftPlan plan;
     int nkx, nzk, NZS, NQ, KT, NL;
     int rf; // use the register-resident FFT when available
+    int nkxg; // register z forward: kx groups per (field, l)
 };
 
 // x stages (FFT over kx) and the cross product.
@@ -38,6 +40,7 @@ struct XArgs {
     FftPlan plan;
     int nkx, NL, NZ2, P;
     int rf;
+    int pair; // product: rows of the register FFT on lane pairs (even factors)
     double *phys; // reference-layout physical field for the single-field stages
     // product a x b: input fields a0 a1 a2 b0 b1 b2 and the three output fields
     const void *fin[6];

@@ -153,4 +153,85 @@ __device__ inline void two_pass(cpx<R> (&x)[M], int t, const cpx<R> *tw, cpx<R> 
     sync();
 }
 
+
+// ------------------------------------------------------- lane-pair DFTs (2H points) --
+// The two lanes of a pair (lane ^ 1) hold H points each of one DFT of length 2H; a radix-2
+// butterfly across the pair (DPP swap of adjacent lanes) joins two register DFTs of length H.
+__device__ inline float pair_swap(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ inline double pair_swap(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <typename R> __device__ inline cpx<R> pair_swap(cpx<R> v) { return {pair_swap(v.x), pair_swap(v.y)}; }
+// component-wise select (a select of whole structs can be lowered through scratch memory)
+template <typename R> __device__ inline cpx<R> csel(bool c, cpx<R> a, cpx<R> b) {
+    return {c ? a.x : b.x, c ? a.y : b.y};
+}
+
+// DIT: lane h holds x[2m + h], m < H; on return it holds X[H h + k], k < H.
+template <typename R, int H, int S> __device__ inline void pair_dit(cpx<R> (&v)[H], int h) {
+    RDft<R, H, S>::run(v);
+    sfor<0, H>([&](auto k) {
+        constexpr int K = decltype(k)::value;
+        v[K] = csel(h != 0, twmul<R, K, 2 * H, S>(v[K]), v[K]);
+    });
+    sfor<0, H>([&](auto k) {
+        constexpr int K = decltype(k)::value;
+        const cpx<R> p = pair_swap(v[K]);
+        v[K] = csel(h != 0, p - v[K], v[K] + p);
+    });
+}
+// DIF: lane h holds x[H h + m], m < H; on return it holds X[2k + h], k < H.
+template <typename R, int H, int S> __device__ inline void pair_dif(cpx<R> (&v)[H], int h) {
+    sfor<0, H>([&](auto m) {
+        constexpr int M = decltype(m)::value;
+        const cpx<R> p = pair_swap(v[M]);
+        v[M] = csel(h != 0, twmul<R, M, 2 * H, S>(p - v[M]), v[M] + p);
+    });
+    RDft<R, H, S>::run(v);
+}
+
+// Two-pass FFT of N = N1 N2 (N1, N2 even) on lane pairs: row thread t (pass A: t < N1, pass B:
+// t < N2) is the pair (t, h).  Pass A input: DIT layout x[m] = line[t + N1 (2m + h)] or, with
+// AIN_HALVES, line[t + N1 (H2 h + m)].  Pass B output: x[k] = X[t + N2 (H1 h + k)], which is the
+// halves input layout of the transposed plan.  x has max(N1, N2) / 2 entries.
+template <typename R, int N1, int N2, int S, bool AIN_HALVES, int M, typename Sync>
+__device__ inline void two_pass_pair(cpx<R> (&x)[M], int t, int h, const cpx<R> *tw, cpx<R> *T, Sync sync) {
+    constexpr int N = N1 * N2, PA = N2 + 1, H1 = N1 / 2, H2 = N2 / 2;
+    static_assert(N1 % 2 == 0 && N2 % 2 == 0, "pair FFT needs even factors");
+    if (t < N1) {
+        cpx<R> a[H2];
+        sfor<0, H2>([&](auto j) { a[j] = x[j]; });
+        int k2base, k2step;
+        if constexpr (AIN_HALVES) {
+            pair_dif<R, H2, S>(a, h);
+            k2base = h;
+            k2step = 2;
+        } else {
+            pair_dit<R, H2, S>(a, h);
+            k2base = H2 * h;
+            k2step = 1;
+        }
+        sfor<0, H2>([&](auto k) {
+            constexpr int K = decltype(k)::value;
+            const int k2 = k2base + k2step * K;
+            cpx<R> w = tw[(t * k2) % N];
+            if (S > 0) w.y = -w.y;
+            const cpx<R> v = a[K];
+            T[t * PA + k2] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+        });
+    }
+    sync();
+    if (t < N2) {
+        cpx<R> b[H1];
+        sfor<0, H1>([&](auto m) { b[m] = T[(2 * decltype(m)::value + h) * PA + t]; });
+        pair_dit<R, H1, S>(b, h);
+        sfor<0, H1>([&](auto k) { x[k] = b[k]; });
+    }
+    sync();
+}
+
 } // namespace cdns

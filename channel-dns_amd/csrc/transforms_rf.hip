@@ -109,6 +109,83 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         }
 }
 
+// ---------------------------------------------- x stage + product, lane pairs --
+// Same work as k_xprod_rf for even N1, N2, with each row of the two-pass FFT on a lane pair
+// (radix-2 butterfly across the pair), so a lane holds half as many points: a group of 2 max(N1,N2)
+// lanes per (l, iz) point.  This halves the registers of the three physical lines that the
+// cross product needs at once (the FP64 kernel then runs several waves per SIMD).
+template <int N1, int N2> struct PGeo {
+    static constexpr int N = N1 * N2;
+    static constexpr int MX = N1 > N2 ? N1 : N2, MH = MX / 2;
+    static constexpr int G = 2 * MX <= 16 ? 16 : (2 * MX <= 32 ? 32 : 64);
+    static constexpr int NG = 64 / G;
+    static constexpr int LB = Geo<N1, N2>::LB;
+};
+
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_xprod_pair(XArgs a) {
+    using Gm = PGeo<N1, N2>;
+    constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, LB = Gm::LB, H1 = N1 / 2, H2 = N2 / 2;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    const int np = a.NL * a.NZ2, K = a.nkx - 1;
+    const int p = blockIdx.x * NG + g;
+    const bool valid = p < np;
+    cpx<R> *B = buf[g];
+    cpx<R> ph[3][MH];
+    sfor<0, 3>([&](auto line) {
+        constexpr int LN = decltype(line)::value;
+        cpx<R> x[MH];
+        const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]) + (size_t)p * a.nkx;
+        const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]) + (size_t)p * a.nkx;
+        sfor<0, MH>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+        if (valid && t < N1)
+            sfor<0, H2>([&](auto m) { x[m] = herm_pack(A, Bv, t + N1 * (2 * decltype(m)::value + h), N, K); });
+        two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, B, wsync);
+        sfor<0, MH>([&](auto j) { ph[LN][j] = x[j]; });
+    });
+    // u x w at the points n = t + N2 (H1 h + k) held by this lane
+    cpx<R> F[MH], Fz[MH];
+    sfor<0, MH>([&](auto j) {
+        F[j] = {R(0), R(0)};
+        Fz[j] = {R(0), R(0)};
+    });
+    if (t < N2) {
+        sfor<0, H1>([&](auto k) {
+            const R u = ph[0][k].x, v = ph[0][k].y, w = ph[1][k].x, ox = ph[1][k].y, oy = ph[2][k].x, oz = ph[2][k].y;
+            F[k] = {v * oz - w * oy, w * ox - u * oz};
+            Fz[k] = {u * oy - v * ox, R(0)};
+        });
+    }
+    const R half = R(0.5) / R(N);
+    cpx<R> *Nx = reinterpret_cast<cpx<R> *>(a.fout[0]), *Ny = reinterpret_cast<cpx<R> *>(a.fout[1]),
+           *Nz = reinterpret_cast<cpx<R> *>(a.fout[2]);
+    // forward of (Nx + i Ny): transposed plan, halves input layout already in place
+    two_pass_pair<R, N2, N1, -1, true>(F, t, h, tw, B, wsync);
+    if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = F[k]; });
+    wsync();
+    if (valid)
+        for (int k = r; k <= K; k += G) {
+            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+            const cpx<R> s = f + gm, d = f - gm;
+            Nx[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            Ny[(size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
+        }
+    wsync();
+    two_pass_pair<R, N2, N1, -1, true>(Fz, t, h, tw, B, wsync);
+    if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = Fz[k]; });
+    wsync();
+    if (valid)
+        for (int k = r; k <= K; k += G) {
+            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+            const cpx<R> s = f + gm;
+            Nz[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+        }
+}
+
 // ------------------------------------------- single-field x stages (physical) --
 // Two points share one complex line: a + i b.
 template <typename R, int N1, int N2>
@@ -189,9 +266,11 @@ __global__ void __launch_bounds__(64) k_xfwd_phys_rf(XArgs a) {
 }
 
 // ------------------------------------------------------------------ z stages --
-// A block owns KB (16, or 8 for the largest lengths) consecutive kx lines of one (field, l), one
-// lane group per line.  The kx-contiguous side (Z[l][iz][kx]) goes through an LDS tile [iz][kx]
-// so that HBM sees whole KB-line rows (256 B in FP64) instead of one 16-byte element per line.
+// Backward: a block owns KB (16, or 8 for the largest lengths) consecutive kx lines of one
+// (field, l), one lane group per line; its output (Z[l][iz][kx], kx contiguous) goes through an LDS
+// tile [iz][kx] so that HBM sees whole KB-line rows (256 B in FP64).  Forward: in FP64 one
+// wavefront per block with lines read straight from HBM (a staged 50 KB tile serialises the loads
+// before the FFT at two blocks per CU); in FP32 the tiled form below.
 template <typename R, int N1, int N2> struct ZTile {
     using Gm = Geo<N1, N2>;
     static constexpr int N = Gm::N;
@@ -249,9 +328,10 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
     }
 }
 
-// forward: Z[l][iz][kx] tile -> FFT over iz per kx line, keep the retained kz, x 1/NZ2
+// forward, tiled: the block's KB lines are read as whole [iz][kx] rows into LDS first (pays off
+// in FP32, where the tile is small and several blocks share a CU).
 template <typename R, int N1, int N2>
-__global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_rf(ZArgs a) {
+__global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_tile(ZArgs a) {
     using Gm = Geo<N1, N2>;
     using T = ZTile<R, N1, N2>;
     constexpr int kZLines = T::KB;
@@ -281,6 +361,39 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_rf(ZArgs a) {
         cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
         sfor<0, N1>([&](auto k1) {
             const int s = t + N2 * decltype(k1)::value;
+            if (s <= NZS) Y[s + NZS] = inv * x[k1];
+            if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
+        });
+    }
+}
+
+// forward: line (f, l, kx) over iz; keep the retained kz, x 1/NZ2.
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
+    using Gm = Geo<N1, N2>;
+    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> buf[NG][LB];
+    const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+    load_roots<R, N>(tw, a.tw);
+    __syncthreads();
+    // neighbouring kx groups of one (field, l) read the same cache lines: keep them on one XCD
+    const int L = xcd_block((int)gridDim.x);
+    const int kxg = L % a.nkxg, lf = L / a.nkxg, l = lf % a.NL, f = lf / a.NL;
+    const int kx = kxg * NG + g;
+    const bool valid = kx < a.nkx;
+    const int NZS = a.NZS;
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx +
+                      (valid ? kx : 0);
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[(size_t)(t + N1 * decltype(n2)::value) * a.nkx]; });
+    two_pass<R, N1, N2, -1>(x, t, tw, buf[g], wsync);
+    if (valid && t < N2) {
+        const R inv = R(1.0 / N);
+        cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
+        sfor<0, N1>([&](auto k1) {
+            const int s = t + N2 * decltype(k1)::value;
             // when N = 2 NZS the slot NZS serves both kz = NZS and kz = -NZS
             if (s <= NZS) Y[s + NZS] = inv * x[k1];
             if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
@@ -290,6 +403,14 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_rf(ZArgs a) {
 
 // ------------------------------------------------------------------ launch --
 template <typename R, int N1, int N2> static void go_xprod(const XArgs &a, hipStream_t st) {
+    if constexpr (N1 % 2 == 0 && N2 % 2 == 0) {
+        if (a.pair) {
+            using Gm = PGeo<N1, N2>;
+            hipLaunchKernelGGL((k_xprod_pair<R, N1, N2>), dim3((a.NL * a.NZ2 + Gm::NG - 1) / Gm::NG), dim3(64), 0, st,
+                               a);
+            return;
+        }
+    }
     using Gm = Geo<N1, N2>;
     hipLaunchKernelGGL((k_xprod_rf<R, N1, N2>), dim3((a.NL * a.NZ2 + Gm::NG - 1) / Gm::NG), dim3(64), 0, st, a);
 }
@@ -309,9 +430,16 @@ template <typename R, int N1, int N2> static void go_zbwd(const ZArgs &a, int nf
                        st, a);
 }
 template <typename R, int N1, int N2> static void go_zfwd(const ZArgs &a, int nf, hipStream_t st) {
-    using T = ZTile<R, N1, N2>;
-    hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES), 0,
-                       st, a);
+    if constexpr (sizeof(R) == 4) {
+        using T = ZTile<R, N1, N2>;
+        hipLaunchKernelGGL((k_zfwd_tile<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES),
+                           0, st, a);
+    } else {
+        using Gm = Geo<N1, N2>;
+        ZArgs b = a;
+        b.nkxg = (a.nkx + Gm::NG - 1) / Gm::NG;
+        hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3(b.nkxg * a.NL * nf), dim3(64), 0, st, b);
+    }
 }
 
 // Supported lengths: N = N1 * N2 with N1, N2 <= 16.

@@ -60,9 +60,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
     extern __shared__ unsigned char smem_raw[];
     R *Bt = reinterpret_cast<R *>(smem_raw); // [njp][PITCH]
     R *midc = Bt + (size_t)a.njp * PITCH;    // [njp], middle-row coefficients
-    const int f = blockIdx.z, part = blockIdx.y;
+    // logical block: the parts of one column group are neighbours on one XCD (shared columns)
+    const int L = xcd_block((int)gridDim.x);
+    const int part = L % a.parts, cgf = L / a.parts, cg = cgf % a.ncg, f = cgf / a.ncg;
     const int l0 = part * PITCH;
-    const bool do_mid = a.has_mid && part == gridDim.y - 1;
+    const bool do_mid = a.has_mid && part == a.parts - 1;
     const int NLH = a.NLH, nj = a.nj, rows = a.rows_m;
     for (int e = threadIdx.x; e < a.njp * PITCH; e += blockDim.x) {
         const int j = e / PITCH, lp = l0 + (e - j * PITCH);
@@ -74,7 +76,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, k = lane >> 4;
-    const int qb = (blockIdx.x * kWaves + wv) * 8 * CT;
+    const int qb = (cg * kWaves + wv) * 8 * CT;
     const double *src[CT];
     bool okq[CT];
 #pragma unroll
@@ -168,7 +170,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
     R *Fe = reinterpret_cast<R *>(smem_raw); // [KL][PITCH]
     R *Fo = Fe + (size_t)a.KL * PITCH;       // [KL][PITCH]
     R *Fm = Fo + (size_t)a.KL * PITCH;       // [PITCH], middle row (even j only)
-    const int f = blockIdx.z, part = blockIdx.y;
+    const int L = xcd_block((int)gridDim.x);
+    const int part = L % a.parts, cgf = L / a.parts, cg = cgf % a.ncg, f = cgf / a.ncg;
     const int m0 = part * PITCH;
     const int rows = a.rows_m, nje = a.nje, njo = a.njo, NLH = a.NLH;
     for (int e = threadIdx.x; e < a.KL * PITCH; e += blockDim.x) {
@@ -184,7 +187,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, k = lane >> 4;
-    const int qb = (blockIdx.x * kWaves + wv) * 8 * CT;
+    const int qb = (cg * kWaves + wv) * 8 * CT;
     const R *Y = reinterpret_cast<const R *>(a.y) + (size_t)f * a.YS * 2;
     const int NQ = a.NQ, NY2 = a.NY2;
     int qa[CT];
@@ -273,17 +276,20 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
 }
 
 // --------------------------------------------------------------------- launch --
-template <typename R, int TRP, int CT> static void go_ybwd(const YArgs &a, int parts, int nf, hipStream_t st) {
+// 1-D grids of (parts x column groups x fields) blocks, decoded by xcd_block.
+template <typename R, int TRP, int CT> static void go_ybwd(YArgs a, int parts, int nf, hipStream_t st) {
     const size_t sm = ((size_t)a.njp * TRP * 16 + a.njp) * sizeof(R);
     const int cols = 8 * CT * kWaves;
-    hipLaunchKernelGGL((k_ybwd_mfma<R, TRP, CT>), dim3((a.NQ + cols - 1) / cols, parts, nf), dim3(64 * kWaves), sm,
-                       st, a);
+    a.parts = parts;
+    a.ncg = (a.NQ + cols - 1) / cols;
+    hipLaunchKernelGGL((k_ybwd_mfma<R, TRP, CT>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * kWaves), sm, st, a);
 }
-template <typename R, int JTP, int CT> static void go_yfwd(const YArgs &a, int parts, int nf, hipStream_t st) {
+template <typename R, int JTP, int CT> static void go_yfwd(YArgs a, int parts, int nf, hipStream_t st) {
     const size_t sm = ((size_t)2 * a.KL * JTP * 16 + JTP * 16) * sizeof(R);
     const int cols = 8 * CT * kWaves;
-    hipLaunchKernelGGL((k_yfwd_mfma<R, JTP, CT>), dim3((a.NQ + cols - 1) / cols, parts, nf), dim3(64 * kWaves), sm,
-                       st, a);
+    a.parts = parts;
+    a.ncg = (a.NQ + cols - 1) / cols;
+    hipLaunchKernelGGL((k_yfwd_mfma<R, JTP, CT>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * kWaves), sm, st, a);
 }
 
 static void y_shape(YArgs &a) {
@@ -310,10 +316,10 @@ bool launch_ybwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
     const int trp = (TR + parts - 1) / parts; // balanced parts; the kernel masks rows >= rows_m
     if (fp32) {
         switch (trp) {
-        case 1: go_ybwd<float, 1, 2>(a, parts, nf, st); break;
-        case 2: go_ybwd<float, 2, 2>(a, parts, nf, st); break;
-        case 3: go_ybwd<float, 3, 2>(a, parts, nf, st); break;
-        default: go_ybwd<float, 4, 2>(a, parts, nf, st); break;
+        case 1: go_ybwd<float, 1, 1>(a, parts, nf, st); break;
+        case 2: go_ybwd<float, 2, 1>(a, parts, nf, st); break;
+        case 3: go_ybwd<float, 3, 1>(a, parts, nf, st); break;
+        default: go_ybwd<float, 4, 1>(a, parts, nf, st); break;
         }
     } else {
         switch (trp) {

@@ -3,7 +3,8 @@
 // Lane i holds 2*CH consecutive coefficients j = 2*CH*i + 2*c + p (slot c, parity p), so each
 // parity chain (j even / j odd, the two decoupled halves of every Chebyshev recurrence) has
 // CH consecutive chain entries per lane.  Sequential recurrences of the reference become
-// affine scans across the chain (log2(64) = 6 shuffle steps).
+// affine scans across the chain (DPP row shifts within 16-lane rows, row totals carried by
+// v_readlane).
 #pragma once
 
 #include "common.hpp"
@@ -12,17 +13,55 @@ namespace cdns {
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
 
-__device__ inline double shdown(double v, int d) { return __shfl_down(v, d, 64); }
-__device__ inline double shup(double v, int d) { return __shfl_up(v, d, 64); }
-__device__ inline double shidx(double v, int s) { return __shfl(v, s, 64); }
-__device__ inline zd shdown(zd v, int d) { return {shdown(v.x, d), shdown(v.y, d)}; }
-__device__ inline zd shup(zd v, int d) { return {shup(v.x, d), shup(v.y, d)}; }
-__device__ inline zd shidx(zd v, int s) { return {shidx(v.x, s), shidx(v.y, s)}; }
+// Cross-lane moves without the LDS crossbar: DPP row shifts (within 16-lane rows) and
+// v_readlane for values at a wave-uniform lane.  GFX9 dpp_ctrl encodings.
+namespace dppc {
+constexpr int row_shl(int n) { return 0x100 + n; }
+constexpr int row_shr(int n) { return 0x110 + n; }
+} // namespace dppc
+
+// lanes without a source lane (shifted past their row) receive `old`
+template <int CTRL> __device__ inline double dpp_mov(double old, double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL> __device__ inline zd dpp_mov(zd old, zd v) {
+    return {dpp_mov<CTRL>(old.x, v.x), dpp_mov<CTRL>(old.y, v.y)};
+}
+// value of v at lane l (l wave-uniform)
+__device__ inline double lane_read(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline zd lane_read(zd v, int l) { return {lane_read(v.x, l), lane_read(v.y, l)}; }
+
+// value at lane - 1 (zero at lane 0) and at lane + 1 (zero at lane 63)
+template <typename T> __device__ inline T lane_prev(T v, int lane) {
+    T o = dpp_mov<dppc::row_shr(1)>(T{}, v);
+    const T r15 = lane_read(v, 15), r31 = lane_read(v, 31), r47 = lane_read(v, 47);
+    if (lane == 16) o = r15;
+    if (lane == 32) o = r31;
+    if (lane == 48) o = r47;
+    return o;
+}
+template <typename T> __device__ inline T lane_next(T v, int lane) {
+    T o = dpp_mov<dppc::row_shl(1)>(T{}, v);
+    const T r16 = lane_read(v, 16), r32 = lane_read(v, 32), r48 = lane_read(v, 48);
+    if (lane == 15) o = r16;
+    if (lane == 31) o = r32;
+    if (lane == 47) o = r48;
+    return o;
+}
 
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    // prefix sums within rows: lane 15 of each row holds the row's sum
+    v += dpp_mov<dppc::row_shr(1)>(0.0, v);
+    v += dpp_mov<dppc::row_shr(2)>(0.0, v);
+    v += dpp_mov<dppc::row_shr(4)>(0.0, v);
+    v += dpp_mov<dppc::row_shr(8)>(0.0, v);
+    return (lane_read(v, 15) + lane_read(v, 31)) + (lane_read(v, 47) + lane_read(v, 63));
 }
 __device__ inline zd wave_sum(zd v) { return {wave_sum(v.x), wave_sum(v.y)}; }
 
@@ -31,19 +70,85 @@ template <int CH> __device__ inline int jof(int lane, int c, int p) { return 2 *
 // Chain neighbours (same parity): value at chain index k+1 / k-1, zero beyond the ends.
 template <int CH, typename T> __device__ inline T chain_next(const T (&v)[CH], int c, int lane) {
     if (c + 1 < CH) return v[c + 1];
-    T o = shdown(v[0], 1);
-    if (lane == 63) o = T{};
-    return o;
+    return lane_next(v[0], lane);
 }
 template <int CH, typename T> __device__ inline T chain_prev(const T (&v)[CH], int c, int lane) {
     if (c > 0) return v[c - 1];
-    T o = shup(v[CH - 1], 1);
-    if (lane == 0) o = T{};
-    return o;
+    return lane_prev(v[CH - 1], lane);
 }
 
 __device__ inline zd axpy(double a, zd x, zd b) { return {b.x + a * x.x, b.y + a * x.y}; }
 __device__ inline double axpy(double a, double x, double b) { return b + a * x; }
+
+// Inclusive scans across the 64 lanes of affine maps y -> B + A y.  Prefix: lane i ends with
+// map_i o ... o map_0; suffix: map_i o ... o map_63.  Hillis-Steele with DPP row shifts inside
+// each 16-lane row, then the row totals (read at wave-uniform lanes) carried across rows.
+template <typename T> __device__ inline void lane_scan_prefix(double &A, T &B, int lane) {
+#define CDNS_SCAN_STEP(D)                                                                                        \
+    {                                                                                                            \
+        const double A2 = dpp_mov<dppc::row_shr(D)>(1.0, A);                                                     \
+        const T B2 = dpp_mov<dppc::row_shr(D)>(T{}, B);                                                          \
+        B = axpy(A, B2, B);                                                                                      \
+        A = A * A2;                                                                                              \
+    }
+    CDNS_SCAN_STEP(1)
+    CDNS_SCAN_STEP(2)
+    CDNS_SCAN_STEP(4)
+    CDNS_SCAN_STEP(8)
+#undef CDNS_SCAN_STEP
+    const double A0 = lane_read(A, 15), A1 = lane_read(A, 31), A2 = lane_read(A, 47);
+    const T B0 = lane_read(B, 15), B1 = lane_read(B, 31), B2 = lane_read(B, 47);
+    const int row = lane >> 4;
+    double Ac = 1.0;
+    T Bc = T{};
+    if (row >= 1) {
+        Ac = A0;
+        Bc = B0;
+    }
+    if (row >= 2) {
+        Bc = axpy(A1, Bc, B1);
+        Ac = A1 * Ac;
+    }
+    if (row >= 3) {
+        Bc = axpy(A2, Bc, B2);
+        Ac = A2 * Ac;
+    }
+    B = axpy(A, Bc, B);
+    A = A * Ac;
+}
+template <typename T> __device__ inline void lane_scan_suffix(double &A, T &B, int lane) {
+#define CDNS_SCAN_STEP(D)                                                                                        \
+    {                                                                                                            \
+        const double A2 = dpp_mov<dppc::row_shl(D)>(1.0, A);                                                     \
+        const T B2 = dpp_mov<dppc::row_shl(D)>(T{}, B);                                                          \
+        B = axpy(A, B2, B);                                                                                      \
+        A = A * A2;                                                                                              \
+    }
+    CDNS_SCAN_STEP(1)
+    CDNS_SCAN_STEP(2)
+    CDNS_SCAN_STEP(4)
+    CDNS_SCAN_STEP(8)
+#undef CDNS_SCAN_STEP
+    const double A1 = lane_read(A, 16), A2 = lane_read(A, 32), A3 = lane_read(A, 48);
+    const T B1 = lane_read(B, 16), B2 = lane_read(B, 32), B3 = lane_read(B, 48);
+    const int row = lane >> 4;
+    double Ac = 1.0;
+    T Bc = T{};
+    if (row <= 2) {
+        Ac = A3;
+        Bc = B3;
+    }
+    if (row <= 1) {
+        Bc = axpy(A2, Bc, B2);
+        Ac = A2 * Ac;
+    }
+    if (row == 0) {
+        Bc = axpy(A1, Bc, B1);
+        Ac = A1 * Ac;
+    }
+    B = axpy(A, Bc, B);
+    A = A * Ac;
+}
 
 // x_k = b_k + a_k x_{k+1}, x beyond the top = 0 (suffix affine scan along one chain).
 template <int CH, typename T>
@@ -55,17 +160,8 @@ __device__ inline void suffix_affine(const double (&a)[CH], const T (&b)[CH], T 
         B = axpy(a[c], B, b[c]);
         A = a[c] * A;
     }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double A2 = shdown(A, d);
-        const T B2 = shdown(B, d);
-        if (lane + d < 64) {
-            B = axpy(A, B2, B);
-            A = A * A2;
-        }
-    }
-    T carry = shdown(B, 1);
-    if (lane == 63) carry = T{};
+    lane_scan_suffix(A, B, lane);
+    T carry = lane_next(B, lane);
 #pragma unroll
     for (int c = CH - 1; c >= 0; --c) {
         carry = axpy(a[c], carry, b[c]);
@@ -83,17 +179,8 @@ __device__ inline void prefix_affine(const double (&a)[CH], const T (&b)[CH], T 
         B = axpy(a[c], B, b[c]);
         A = a[c] * A;
     }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double A2 = shup(A, d);
-        const T B2 = shup(B, d);
-        if (lane >= d) {
-            B = axpy(A, B2, B);
-            A = A * A2;
-        }
-    }
-    T carry = shup(B, 1);
-    if (lane == 0) carry = T{};
+    lane_scan_prefix(A, B, lane);
+    T carry = lane_prev(B, lane);
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         carry = axpy(a[c], carry, b[c]);
@@ -103,21 +190,10 @@ __device__ inline void prefix_affine(const double (&a)[CH], const T (&b)[CH], T 
 
 // Inclusive suffix sum along one chain.
 template <int CH> __device__ inline void suffix_sum(const zd (&b)[CH], zd (&x)[CH], int lane) {
-    zd B = {0.0, 0.0};
+    double a[CH];
 #pragma unroll
-    for (int c = CH - 1; c >= 0; --c) B = B + b[c];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const zd B2 = shdown(B, d);
-        if (lane + d < 64) B = B + B2;
-    }
-    zd carry = shdown(B, 1);
-    if (lane == 63) carry = zd{0.0, 0.0};
-#pragma unroll
-    for (int c = CH - 1; c >= 0; --c) {
-        carry = carry + b[c];
-        x[c] = carry;
-    }
+    for (int c = 0; c < CH; ++c) a[c] = 1.0;
+    suffix_affine<CH, zd>(a, b, x, lane);
 }
 
 // Chebyshev derivative coefficients d = D f (vector_ops.cpp:17-25):
@@ -181,7 +257,7 @@ template <int CH> __device__ inline zd col_at(const zd (&v)[2][CH], int j) {
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp)
             if (cc == c && pp == p) mine = v[pp][cc];
-    return shidx(mine, lane);
+    return lane_read(mine, lane);
 }
 template <int CH> __device__ inline double col_at(const double (&v)[2][CH], int j) {
     const int lane = j / (2 * CH), r = j - lane * 2 * CH, c = r >> 1, p = r & 1;
@@ -191,7 +267,7 @@ template <int CH> __device__ inline double col_at(const double (&v)[2][CH], int 
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp)
             if (cc == c && pp == p) mine = v[pp][cc];
-    return shidx(mine, lane);
+    return lane_read(mine, lane);
 }
 
 } // namespace cdns

@@ -23,16 +23,18 @@ CONFIGS = {
 # Two kernel families per stage: the defaults (register-resident two-pass FFTs for x and z,
 # matrix-core y stages) and the alternates (LDS Stockham FFTs, CDNS_FFT=stockham; vector-ALU
 # y stages, CDNS_YSTAGE=vector).
-FFT_KINDS = ["register", "stockham"]
+FFT_KINDS = ["register", "stockham", "single-lane-rows"]
 
 
 def _use_fft(monkeypatch, kind):
+    for v in ("CDNS_FFT", "CDNS_YSTAGE", "CDNS_XPRODUCT", "CDNS_SOLVER_TABLES"):
+        monkeypatch.delenv(v, raising=False)
     if kind == "stockham":
         monkeypatch.setenv("CDNS_FFT", "stockham")
         monkeypatch.setenv("CDNS_YSTAGE", "vector")
-    else:
-        monkeypatch.delenv("CDNS_FFT", raising=False)
-        monkeypatch.delenv("CDNS_YSTAGE", raising=False)
+        monkeypatch.setenv("CDNS_SOLVER_TABLES", "complex")
+    elif kind == "single-lane-rows":
+        monkeypatch.setenv("CDNS_XPRODUCT", "single")
 
 
 @pytest.mark.parametrize("fft", FFT_KINDS)
@@ -132,7 +134,8 @@ def test_solve_modes(gpu, correct, fuse):
                                                 ("channel_32", 20, 1e-9, "register"),
                                                 ("stage3", 5, 1e-10, "register"),
                                                 ("channel_32", 5, 1e-10, "stockham"),
-                                                ("stage3", 5, 1e-10, "stockham")])
+                                                ("stage3", 5, 1e-10, "stockham"),
+                                                ("channel_32", 5, 1e-10, "single-lane-rows")])
 def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
     _use_fft(monkeypatch, fft)
     cfg = stage3_config() if name == "stage3" else CONFIGS[name]
@@ -145,8 +148,14 @@ def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
     got, want = dev.get_state("velocity"), ref.get_state("velocity")
     errs = fields.rel_l2_components(got, want)
     assert max(errs) < tol, errs
-    for f in ("velocity_prev", "nonlinear_prev", "pressure_bc_prev"):
+    for f in ("velocity_prev", "nonlinear_prev"):
         assert fields.rel_l2(dev.get_state(f), ref.get_state(f)) < 10 * tol, f
+    # the pressure wall condition is a sum over j of terms of the size of N_y that cancels to
+    # far below them (1e-11 for the near-laminar example), so its error is measured against
+    # the size of the summands rather than of the result
+    pd, pr = dev.get_state("pressure_bc_prev"), ref.get_state("pressure_bc_prev")
+    terms = np.abs(ref.get_state("nonlinear_prev")).max() * (cfg["NYS"] + 1)
+    assert np.abs(pd - pr).max() <= 10 * tol * max(np.abs(pr).max(), terms * 1e-3), "pressure_bc_prev"
     assert fields.rel_l2(dev.get_state("pressure"), ref.get_state("pressure")) < 10 * tol
     # constant-flux dpdx = (Q_target - Q(u*)) / Q(u_G) cancels, so its error is measured against
     # the scale 1/Q(u_G) of one unit of flow-rate error (simulation.cpp:187-188)
