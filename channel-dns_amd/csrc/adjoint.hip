@@ -38,7 +38,8 @@ template <int CH> __device__ inline void cheb_dyT(const zd (&g)[2][CH], zd (&o)[
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         const int j0 = jof<CH>(lane, c, 0), j1 = j0 + 1;
-        o[0][c] = j0 <= ny ? (double)(2 * j0) * chain_prev<CH>(PO, c, lane) : zd{0.0, 0.0};
+        const zd prev = chain_prev<CH>(PO, c, lane);
+        o[0][c] = j0 <= ny ? (double)(2 * j0) * prev : zd{0.0, 0.0};
         o[1][c] = j1 <= ny ? (double)(2 * j1) * PE[c] : zd{0.0, 0.0};
     }
 }

@@ -1,120 +1,123 @@
 // Per-(kx,kz) Chebyshev-tau solver of the channel-dns step on gfx950: one 64-lane wavefront
-// per wavenumber pair.
+// per wavenumber pair, the even Chebyshev chain in lanes 0-31 and the odd chain in lanes 32-63.
 //
 // The reference solves each column with sequential y recurrences over 8-lane kx tiles
 // (EqSolvePV::solve_modes / solve_tau_tile, src/solver/eqsolve_pv.cpp:64-128, 311-450).
-// The recurrences are linear in the data, so each becomes an affine scan along the even and
-// odd Chebyshev chains (6 shuffle steps), and the influence-matrix and sigma corrections are
+// The recurrences are linear in the data and couple one parity only, so each becomes one affine
+// scan over the two 32-lane halves at once; the influence-matrix and sigma corrections are
 // lane-parallel over y with wavefront reductions for the wall divergence.
 #include "internal.hpp"
-#include "wave.hpp"
+#include "split.hpp"
 
 namespace cdns {
 
 __device__ inline zd cdivr(zd z, double d) { return {z.x / d, z.y / d}; }
 
-// solve_tau_column (eqsolve_pv.cpp:16-60) for one column held by the wavefront.
-template <int CH>
-__device__ void tau_solve(const zd (&r)[2][CH], zd (&out)[2][CH], const double (&ad)[2][CH],
-                          const double (&WL)[2][CH], const double (&WD)[2][CH], const double (&WR)[2][CH],
-                          int lane, int ny, double w, double g, double a, double b, bool zero_mode, zd lower,
-                          zd upper) {
-    const bool normalize = g == 0.0 && a == 0.0 && zero_mode;
-    zd R[2][CH];
+// solve_tau_column (eqsolve_pv.cpp:16-60) for NR right-hand sides that share the operator
+// (wavenumber w, ad, WL/WD/WR) and differ in wall data and closure weights (a, b).  The two
+// sweeps and the closure accumulators are affine scans whose multipliers depend on the operator
+// only, so the NR right-hand sides ride one scan each; the closure rows j = 0 and j = 1 sit in
+// entry 0 of lanes 0 and 32.
+template <int NE, int NR>
+__device__ void tau_solve(const zd (&r)[NR][NE], zd (&out)[NR][NE], const double (&ad)[NE], const double (&WL)[NE],
+                          const double (&WD)[NE], const double (&WR)[NE], int lane, int ny, double w, double g,
+                          const double (&a)[NR], const double (&b)[NR], bool zero_mode, const zd (&lower)[NR],
+                          const zd (&upper)[NR]) {
+    const int p = sp::parity(lane);
+    const bool head = (lane & 31) == 0; // entry 0 of this lane is j = p
+    zd R[NR][NE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int k = 0; k < NR; ++k)
 #pragma unroll
-        for (int c = 0; c < CH; ++c) R[p][c] = r[p][c];
-    if (normalize) {
-        // compatibility correction of the pure-Neumann mean pressure (eqsolve_pv.cpp:19-27)
-        zd corr = {0.0, 0.0};
+        for (int e = 0; e < NE; ++e) R[k][e] = r[k][e];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, 0);
-            if (j >= 2 && j <= ny) corr = corr + cdivr(r[0][c], (double)(j * j - 1));
+    for (int k = 0; k < NR; ++k) {
+        if (g == 0.0 && a[k] == 0.0 && zero_mode) {
+            // compatibility correction of the pure-Neumann mean pressure (eqsolve_pv.cpp:19-27)
+            zd corr = {0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const int j = sp::jof<NE>(lane, e);
+                if (p == 0 && j >= 2 && j <= ny) corr = corr + cdivr(r[k][e], (double)(j * j - 1));
+            }
+            corr = wave_sum(corr);
+            if (b[k] != 0.0) corr = corr + cdivr(lower[k] + upper[k], 2.0 * b[k]);
+            if (lane == 0) R[k][0] = corr;
         }
-        corr = wave_sum(corr);
-        if (b != 0.0) corr = corr + cdivr(lower + upper, 2.0 * b);
-        if (lane == 0) R[0][0] = corr;
     }
     // tau right-hand side and the two closure rows (eqsolve_pv.cpp:28-33)
-    zd o[2][CH];
-    double adn[2][CH], WLn[2][CH];
+    double adn[NE], WLn[NE];
+    zd o[NR][NE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int e = 0; e < NE; ++e) {
+        adn[e] = sp::chain_next<NE>(ad, e, lane);
+        WLn[e] = sp::chain_next<NE>(WL, e, lane);
+        const int j = sp::jof<NE>(lane, e);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, p);
-            const zd rm = chain_prev<CH>(R[p], c, lane), rn = chain_next<CH>(R[p], c, lane);
-            adn[p][c] = chain_next<CH>(ad[p], c, lane);
-            WLn[p][c] = chain_next<CH>(WL[p], c, lane);
-            if (j == 0) o[p][c] = cdivr(lower + upper, 2.0);
-            else if (j == 1) o[p][c] = cdivr(lower - upper, 2.0);
-            else if (j <= ny) o[p][c] = (WL[p][c] * rm + WD[p][c] * R[p][c]) + WR[p][c] * rn;
-            else o[p][c] = {0.0, 0.0};
+        for (int k = 0; k < NR; ++k) {
+            const zd rm = sp::chain_prev<NE>(R[k], e, lane), rn = sp::chain_next<NE>(R[k], e, lane);
+            const zd closure = p ? cdivr(lower[k] - upper[k], 2.0) : cdivr(lower[k] + upper[k], 2.0);
+            if (e == 0 && head) o[k][e] = closure;
+            else if (j <= ny) o[k][e] = (WL[e] * rm + WD[e] * R[k][e]) + WR[e] * rn;
+            else o[k][e] = {0.0, 0.0};
         }
-    // backward sweep out(j) -= WR(j) w ad(j+2) out(j+2), 2 <= j <= ny-2 (eqsolve_pv.cpp:34-35)
-    zd x[2][CH];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        double m[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, p);
-            m[c] = (j >= 2 && j <= ny - 2) ? -(WR[p][c] * w * adn[p][c]) : 0.0;
-        }
-        suffix_affine<CH, zd>(m, o[p], x[p], lane);
     }
-    // closure accumulators ad01 (eqsolve_pv.cpp:37-50): A(j) = f(j) - WL(j+2) w ad(j+2) A(j+2)
-    double A[2][CH];
+    // backward sweep out(j) -= WR(j) w ad(j+2) out(j+2), 2 <= j <= ny-2 (eqsolve_pv.cpp:34-35), and
+    // the closure accumulators ad01 (eqsolve_pv.cpp:37-50): A(j) = f(j) - WL(j+2) w ad(j+2) A(j+2)
+    zd x[NR][NE];
+    double A[NR][NE];
+    {
+        double m[NE], beta[NE], f[NR][NE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        double f[CH], beta[CH];
+        for (int e = 0; e < NE; ++e) {
+            const int j = sp::jof<NE>(lane, e);
+            m[e] = (j >= 2 && j <= ny - 2) ? -(WR[e] * w * adn[e]) : 0.0;
+            beta[e] = (j <= ny - 2) ? -(WLn[e] * w * adn[e]) : 0.0;
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, p);
-            const double fj = (a == 0.0 && b == 0.0) ? (double)((1 - p) * j * j + p) : a + b * j * j;
-            f[c] = j <= ny ? fj : 0.0;
-            beta[c] = (j <= ny - 2) ? -(WLn[p][c] * w * adn[p][c]) : 0.0;
-        }
-        suffix_affine<CH, double>(beta, f, A[p], lane);
-    }
-    zd S[2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        zd s = {0.0, 0.0};
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, p);
-            if (j >= 2 && j <= ny) s = s + (A[p][c] * ad[p][c]) * x[p][c];
-        }
-        S[p] = wave_sum(s);
-    }
-    const double A0 = col_at<CH>(A, 0), A1 = col_at<CH>(A, 1);
-    const zd x0 = col_at<CH>(x, 0), x1 = col_at<CH>(x, 1);
-    const zd out0 = normalize ? zd{0.0, 0.0} : cdivr(x0 - S[0], A0);
-    const zd out1 = cdivr(x1 - S[1], A1);
-    // forward sweep out(j) = (out(j) - WL(j) w out(j-2)) ad(j) (eqsolve_pv.cpp:58-59)
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        double aa[CH];
-        zd bb[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = jof<CH>(lane, c, p);
-            if (j < 2) {
-                aa[c] = 0.0;
-                bb[c] = p ? out1 : out0;
-            } else if (j <= ny) {
-                aa[c] = -(WL[p][c] * w * ad[p][c]);
-                bb[c] = ad[p][c] * x[p][c];
-            } else {
-                aa[c] = 0.0;
-                bb[c] = {0.0, 0.0};
+            for (int k = 0; k < NR; ++k) {
+                const double fj = (a[k] == 0.0 && b[k] == 0.0) ? (double)((1 - p) * j * j + p) : a[k] + b[k] * j * j;
+                f[k][e] = j <= ny ? fj : 0.0;
             }
         }
-        prefix_affine<CH, zd>(aa, bb, out[p], lane);
+        sp::suffix_affine_c<NE, NR>(m, o, x, lane);
+        sp::suffix_affine_n<NE, NR>(beta, f, A, lane);
     }
+    // closure sums over j >= 2 of this chain
+    double S[NR][2];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        zd s = {0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int j = sp::jof<NE>(lane, e);
+            if (j >= 2 && j <= ny) s = s + (A[k][e] * ad[e]) * x[k][e];
+        }
+        S[k][0] = s.x;
+        S[k][1] = s.y;
+    }
+    sp::half_sum<2 * NR>(reinterpret_cast<double(&)[2 * NR]>(S));
+    // forward sweep out(j) = (out(j) - WL(j) w out(j-2)) ad(j) (eqsolve_pv.cpp:58-59) from the
+    // closure values out(0), out(1) (eqsolve_pv.cpp:51-57)
+    double aa[NE];
+    zd bb[NR][NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const int j = sp::jof<NE>(lane, e);
+        aa[e] = (j >= 2 && j <= ny) ? -(WL[e] * w * ad[e]) : 0.0;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (e == 0 && head) {
+                const bool normalize = p == 0 && g == 0.0 && a[k] == 0.0 && zero_mode;
+                const double inv = 1.0 / A[k][0];
+                bb[k][e] = normalize ? zd{0.0, 0.0} : zd{(x[k][0].x - S[k][0]) * inv, (x[k][0].y - S[k][1]) * inv};
+            } else if (j <= ny) {
+                bb[k][e] = ad[e] * x[k][e];
+            } else {
+                bb[k][e] = {0.0, 0.0};
+            }
+        }
+    }
+    sp::prefix_affine_c<NE, NR>(aa, bb, out, lane);
 }
 
 // Fused coupled solve of one (kx,kz) pair: EqSolvePV::solve_modes (eqsolve_pv.cpp:311-450),
@@ -122,117 +125,100 @@ __device__ void tau_solve(const zd (&r)[2][CH], zd (&out)[2][CH], const double (
 // (timestepper.cpp:55-72), correct_zero_mode (timestepper.cpp:89-97) and correct_flow
 // (simulation.cpp:173-202) fused in MODE_STEP; the setup problems of eqsolve_pv.cpp:266-303
 // and 452-477 in MODE_INFLUENCE / MODE_BPROBLEM.
-template <int CH>
+template <int NE, int MODE>
 __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
-    const int lane = lane_id();
+    const int lane = lane_id(), p = sp::parity(lane);
     const int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (q >= a.NQ) return;
-    const int ny = a.ny, mode = a.mode;
+    const int ny = a.ny;
+    constexpr int mode = MODE;
     const int kxi = q / a.nzk, kzi = q - kxi * a.nzk;
     const bool zero = kxi == 0 && 2 * kzi == a.nzk - 1;
     const double kx = a.WAVX[kxi], kz = a.WAVZ[kzi];
     const size_t off = (size_t)q * a.JS;
-    double WL[2][CH], WD[2][CH], WR[2][CH], adp[2][CH], adv[2][CH];
-    load_rcol<CH>(a.WL, lane, ny, WL);
-    load_rcol<CH>(a.WD, lane, ny, WD);
-    load_rcol<CH>(a.WR, lane, ny, WR);
-    load_rcol<CH>(a.ad_p + off, lane, ny, adp);
-    load_rcol<CH>(a.ad_v + off, lane, ny, adv);
+    double WL[NE], WD[NE], WR[NE], adp[NE], adv[NE];
+    sp::load_rcol<NE>(a.WL, lane, ny, WL);
+    sp::load_rcol<NE>(a.WD, lane, ny, WD);
+    sp::load_rcol<NE>(a.WR, lane, ny, WR);
+    sp::load_rcol<NE>(a.ad_p + off, lane, ny, adp);
+    sp::load_rcol<NE>(a.ad_v + off, lane, ny, adv);
 
     // ---- velocity right-hand side
-    zd rv[3][2][CH];
-    zd nly[2][CH];
-    if (mode == MODE_STEP) {
+    zd rv[3][NE];
+    zd nly[NE];
+    if constexpr (mode == MODE_STEP) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            zd u[2][CH], up[2][CH], n[2][CH], np[2][CH];
-            load_col<CH>(a.u[k] + off, lane, ny, u);
-            load_col<CH>(a.up[k] + off, lane, ny, up);
-            load_col<CH>(a.nl[k] + off, lane, ny, n);
-            load_col<CH>(a.nlp[k] + off, lane, ny, np);
+            zd u[NE], up[NE], n[NE], np[NE];
+            sp::load_col<NE>(a.u[k] + off, lane, ny, u);
+            sp::load_col<NE>(a.up[k] + off, lane, ny, up);
+            sp::load_col<NE>(a.nl[k] + off, lane, ny, n);
+            sp::load_col<NE>(a.nlp[k] + off, lane, ny, np);
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int c = 0; c < CH; ++c) {
-                    rv[k][p][c] = ((a.a0 * u[p][c] + a.a1 * up[p][c]) + a.b0 * n[p][c]) + a.b1 * np[p][c];
-                    if (k == 1) nly[p][c] = n[p][c];
-                }
+            for (int e = 0; e < NE; ++e) {
+                rv[k][e] = ((a.a0 * u[e] + a.a1 * up[e]) + a.b0 * n[e]) + a.b1 * np[e];
+                if (k == 1) nly[e] = n[e];
+            }
         }
-    } else if (mode == MODE_GIVEN) {
+    } else if constexpr (mode == MODE_GIVEN) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) load_col<CH>(a.rv[k] + off, lane, ny, rv[k]);
+        for (int k = 0; k < 3; ++k) sp::load_col<NE>(a.rv[k] + off, lane, ny, rv[k]);
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int c = 0; c < CH; ++c) rv[k][p][c] = {0.0, 0.0};
+            for (int e = 0; e < NE; ++e) rv[k][e] = {0.0, 0.0};
     }
 
     // ---- pressure right-hand side
-    zd rp[2][CH];
+    zd rp[1][NE];
     if (mode == MODE_STEP || (mode == MODE_GIVEN && a.rp == nullptr)) {
-        zd d1[2][CH];
-        cheb_dy<CH>(rv[1], d1, lane, ny);
+        zd d1[NE];
+        sp::cheb_dy<NE>(rv[1], d1, lane, ny);
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c)
-                rp[p][c] = ((zd{0.0, 0.0} + ikmul(kx, rv[0][p][c])) + d1[p][c]) + ikmul(kz, rv[2][p][c]);
+        for (int e = 0; e < NE; ++e) rp[0][e] = ((zd{0.0, 0.0} + ikmul(kx, rv[0][e])) + d1[e]) + ikmul(kz, rv[2][e]);
     } else if (mode == MODE_GIVEN) {
-        load_col<CH>(a.rp + off, lane, ny, rp);
+        sp::load_col<NE>(a.rp + off, lane, ny, rp[0]);
     } else if (mode == MODE_BPROBLEM) {
         // r_0 = NYS-1, r_j = 2 (NYS - ((j+1) mod 2)) for 1 <= j < NYS (eqsolve_pv.cpp:455-460)
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const int j = jof<CH>(lane, c, p);
-                double v = 0.0;
-                if (j == 0) v = (double)(ny - 1);
-                else if (j < ny) v = (double)(2 * (ny - ((j + 1) % 2)));
-                rp[p][c] = {v, 0.0};
-            }
+        for (int e = 0; e < NE; ++e) {
+            const int j = sp::jof<NE>(lane, e);
+            double v = 0.0;
+            if (j == 0) v = (double)(ny - 1);
+            else if (j < ny) v = (double)(2 * (ny - ((j + 1) % 2)));
+            rp[0][e] = {v, 0.0};
+        }
     } else {
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) rp[p][c] = {0.0, 0.0};
+        for (int e = 0; e < NE; ++e) rp[0][e] = {0.0, 0.0};
     }
 
     // ---- boundary data
-    zd plo = {0.0, 0.0}, pup = {0.0, 0.0};
-    if (mode == MODE_STEP) {
-        zd sn0 = {0.0, 0.0}, sn1 = {0.0, 0.0};
+    zd plo[1] = {{0.0, 0.0}}, pup[1] = {{0.0, 0.0}};
+    if constexpr (mode == MODE_STEP) {
+        // wall sums of N_y: sum_j N_j at y = +1 and sum_j (-1)^(j+1) N_j at y = -1
+        zd sn0 = {0.0, 0.0};
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const int j = jof<CH>(lane, c, p);
-                if (j <= ny) {
-                    sn0 = sn0 + nly[p][c];
-                    sn1 = p == 0 ? sn1 - nly[p][c] : sn1 + nly[p][c];
-                }
-            }
-        sn0 = wave_sum(sn0);
-        sn1 = wave_sum(sn1);
-        const zd B0 = a.bcw[(size_t)q * 2 + 0] + sn0, B1 = a.bcw[(size_t)q * 2 + 1] + sn1;
+        for (int e = 0; e < NE; ++e) sn0 = sn0 + nly[e];
+        double t[4] = {sn0.x, sn0.y, p ? sn0.x : -sn0.x, p ? sn0.y : -sn0.y};
+        wave_sum<4>(t);
+        const zd B0 = a.bcw[(size_t)q * 2 + 0] + zd{t[0], t[1]}, B1 = a.bcw[(size_t)q * 2 + 1] + zd{t[2], t[3]};
         const zd P0 = a.bc_prev[(size_t)q * 2 + 0], P1 = a.bc_prev[(size_t)q * 2 + 1];
-        plo = a.b0 * B0 + a.b1 * P0;
-        pup = a.b0 * B1 + a.b1 * P1;
+        plo[0] = a.b0 * B0 + a.b1 * P0;
+        pup[0] = a.b0 * B1 + a.b1 * P1;
         if (lane == 0) {
             a.bc_prev[(size_t)q * 2 + 0] = B0;
             a.bc_prev[(size_t)q * 2 + 1] = B1;
-            a.bc_ext[(size_t)q * 2 + 0] = plo;
-            a.bc_ext[(size_t)q * 2 + 1] = pup;
+            a.bc_ext[(size_t)q * 2 + 0] = plo[0];
+            a.bc_ext[(size_t)q * 2 + 1] = pup[0];
         }
-    } else if (mode == MODE_GIVEN) {
-        plo = a.pbc[(size_t)q * 2 + 0];
-        pup = a.pbc[(size_t)q * 2 + 1];
-    } else if (mode == MODE_INFLUENCE) {
-        plo = {(double)(1 - a.influence_index), 0.0};
-        pup = {(double)a.influence_index, 0.0};
+    } else if constexpr (mode == MODE_GIVEN) {
+        plo[0] = a.pbc[(size_t)q * 2 + 0];
+        pup[0] = a.pbc[(size_t)q * 2 + 1];
+    } else if constexpr (mode == MODE_INFLUENCE) {
+        plo[0] = {(double)(1 - a.influence_index), 0.0};
+        pup[0] = {(double)a.influence_index, 0.0};
     }
     zd vlo[3], vup[3];
 #pragma unroll
@@ -249,192 +235,176 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
     const double wn_v = -(kx * kx + kz * kz + a.g_v);
 
     // ---- pressure, its y derivative, the three velocity components
-    zd pc[2][CH], dy[2][CH];
-    tau_solve<CH>(rp, pc, adp, WL, WD, WR, lane, ny, wn_p, a.G_pre, a.pa, a.pb, zero, plo, pup);
-    cheb_dy<CH>(pc, dy, lane, ny);
-    const zd tail0 = col_at<CH>(rv[1], ny), tail1 = col_at<CH>(rv[1], ny - 1);
-    zd vc[3][2][CH];
+    zd pc[1][NE];
+    {
+        const double pa[1] = {a.pa}, pb[1] = {a.pb};
+        tau_solve<NE, 1>(rp, pc, adp, WL, WD, WR, lane, ny, wn_p, a.G_pre, pa, pb, zero, plo, pup);
+    }
+    const zd tail0 = sp::col_at<NE>(rv[1], ny), tail1 = sp::col_at<NE>(rv[1], ny - 1);
+    zd vc[3][NE];
+    {
+        zd dy[NE], forcing[3][NE];
+        sp::cheb_dy<NE>(pc[0], dy, lane, ny);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        zd forcing[2][CH];
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const zd grad = k == 0 ? ikmul(kx, pc[p][c]) : (k == 1 ? dy[p][c] : ikmul(kz, pc[p][c]));
-                forcing[p][c] = a.Re * (grad - rv[k][p][c]);
+            for (int e = 0; e < NE; ++e) {
+                const zd grad = k == 0 ? ikmul(kx, pc[0][e]) : (k == 1 ? dy[e] : ikmul(kz, pc[0][e]));
+                forcing[k][e] = a.Re * (grad - rv[k][e]);
             }
-        tau_solve<CH>(forcing, vc[k], adv, WL, WD, WR, lane, ny, wn_v, a.g_v, a.aV[k], a.bV[k], zero, vlo[k], vup[k]);
+        const double av[3] = {a.aV[0], a.aV[1], a.aV[2]}, bv[3] = {a.bV[0], a.bV[1], a.bV[2]};
+        tau_solve<NE, 3>(forcing, vc, adv, WL, WD, WR, lane, ny, wn_v, a.g_v, av, bv, zero, vlo, vup);
     }
 
     // ---- wall divergence (eqsolve_pv.cpp:397-406)
     zd div0 = {0.0, 0.0}, div1 = {0.0, 0.0};
     if (a.correct || mode == MODE_INFLUENCE) {
+        zd tang = {0.0, 0.0}, normal = {0.0, 0.0};
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const int j = jof<CH>(lane, c, p);
-                if (j <= ny) {
-                    const zd tang = ikmul(kx, vc[0][p][c]) + ikmul(kz, vc[2][p][c]);
-                    const zd normal = (double)(j * j) * vc[1][p][c];
-                    div0 = div0 + (tang + normal);
-                    div1 = div1 + (p == 0 ? tang - normal : normal - tang);
-                }
-            }
-        div0 = wave_sum(div0);
-        div1 = wave_sum(div1);
+        for (int e = 0; e < NE; ++e) {
+            const int j = sp::jof<NE>(lane, e);
+            tang = tang + (ikmul(kx, vc[0][e]) + ikmul(kz, vc[2][e]));
+            normal = normal + (double)(j * j) * vc[1][e];
+        }
+        // y = +1: tang + normal; y = -1: (-1)^j tang - (-1)^j normal
+        const zd d0 = tang + normal, d1 = p ? normal - tang : tang - normal;
+        double t[4] = {d0.x, d0.y, d1.x, d1.y};
+        wave_sum<4>(t);
+        div0 = {t[0], t[1]};
+        div1 = {t[2], t[3]};
     }
-    if (mode == MODE_INFLUENCE) {
+    if constexpr (mode == MODE_INFLUENCE) {
         if (lane == 0) {
             a.div_out[(size_t)q * 2 + 0] = div0;
             a.div_out[(size_t)q * 2 + 1] = div1;
         }
-        store_col<CH>(a.P + off, lane, ny, pc);
+        sp::store_col<NE>(a.P + off, lane, ny, pc[0]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) store_col<CH>(a.V[k] + off, lane, ny, vc[k]);
+        for (int k = 0; k < 3; ++k) sp::store_col<NE>(a.V[k] + off, lane, ny, vc[k]);
         return;
     }
 
     // ---- influence-matrix and sigma corrections (eqsolve_pv.cpp:407-445)
     zd c0 = {0.0, 0.0}, c1 = {0.0, 0.0}, sig[2] = {{0.0, 0.0}, {0.0, 0.0}};
-    if (a.correct && a.compact && mode == MODE_STEP) {
+    const bool compact = a.compact && mode == MODE_STEP;
+    if (a.correct) {
         const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
         c0 = cmul(div1, I1[0]) - cmul(div0, I1[1]);
         c1 = cmul(div0, I0[1]) - cmul(div1, I0[0]);
         if (a.sigma) {
-            const zd ptop = col_at<CH>(pc, ny) + (a.PIr[0][off + ny] * c0 + a.PIr[1][off + ny] * c1);
-            const zd vt0 = col_at<CH>(vc[1], ny) + (a.VIv[0][off + ny] * c0 + a.VIv[1][off + ny] * c1);
-            const zd vt1 = col_at<CH>(vc[1], ny - 1) + (a.VIv[0][off + ny - 1] * c0 + a.VIv[1][off + ny - 1] * c1);
-            sig[0] = cdiv((-wn_v) * vt0 - a.Re * tail0, a.SIG0[(size_t)q * 2 + 0]);
-            sig[1] = cdiv((-wn_v) * vt1 + a.Re * ((2.0 * ny) * ptop - tail1), a.SIG0[(size_t)q * 2 + 1]);
-        }
-    } else if (a.correct) {
-        const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
-        c0 = cmul(div1, I1[0]) - cmul(div0, I1[1]);
-        c1 = cmul(div0, I0[1]) - cmul(div1, I0[0]);
-        if (a.sigma) {
-            const zd ptop = col_at<CH>(pc, ny) + (cmul(c0, a.PI[0][off + ny]) + cmul(c1, a.PI[1][off + ny]));
-            const zd vt0 = col_at<CH>(vc[1], ny) + (cmul(c0, a.VI[0][1][off + ny]) + cmul(c1, a.VI[1][1][off + ny]));
-            const zd vt1 = col_at<CH>(vc[1], ny - 1) +
-                           (cmul(c0, a.VI[0][1][off + ny - 1]) + cmul(c1, a.VI[1][1][off + ny - 1]));
+            zd ptop, vt0, vt1;
+            if (compact) {
+                ptop = sp::col_at<NE>(pc[0], ny) + (a.PIr[0][off + ny] * c0 + a.PIr[1][off + ny] * c1);
+                vt0 = sp::col_at<NE>(vc[1], ny) + (a.VIv[0][off + ny] * c0 + a.VIv[1][off + ny] * c1);
+                vt1 = sp::col_at<NE>(vc[1], ny - 1) + (a.VIv[0][off + ny - 1] * c0 + a.VIv[1][off + ny - 1] * c1);
+            } else {
+                ptop = sp::col_at<NE>(pc[0], ny) + (cmul(c0, a.PI[0][off + ny]) + cmul(c1, a.PI[1][off + ny]));
+                vt0 = sp::col_at<NE>(vc[1], ny) + (cmul(c0, a.VI[0][1][off + ny]) + cmul(c1, a.VI[1][1][off + ny]));
+                vt1 = sp::col_at<NE>(vc[1], ny - 1) +
+                      (cmul(c0, a.VI[0][1][off + ny - 1]) + cmul(c1, a.VI[1][1][off + ny - 1]));
+            }
             sig[0] = cdiv((-wn_v) * vt0 - a.Re * tail0, a.SIG0[(size_t)q * 2 + 0]);
             sig[1] = cdiv((-wn_v) * vt1 + a.Re * ((2.0 * ny) * ptop - tail1), a.SIG0[(size_t)q * 2 + 1]);
         }
     }
-    zd P[2][CH], V[3][2][CH];
-    if (a.correct && a.sigma && a.compact && mode == MODE_STEP) {
-        // P += c0 P0 + c1 P1 + s_(1-p) PB;  v += c0 v0 + c1 v1 + s_p vB;
-        // (u, w) += i (kx, kz) (c0 f0 + c1 f1 + s_(1-p) fB)
-        double r0[2][CH], r1[2][CH], rb[2][CH];
-        load_rcol<CH>(a.PIr[0] + off, lane, ny, r0);
-        load_rcol<CH>(a.PIr[1] + off, lane, ny, r1);
-        load_rcol<CH>(a.PBr + off, lane, ny, rb);
+    // the B response enters with sigma_(1 - p) for P, u, w and sigma_p for v (q = (j+1) mod 2)
+    const zd sp_own = p ? sig[1] : sig[0], sp_other = p ? sig[0] : sig[1];
+    zd P[NE], V[3][NE];
+    if (a.correct && a.sigma && compact) {
+        // P += c0 P0 + c1 P1 + s PB;  v += c0 v0 + c1 v1 + s vB;  (u, w) += i (kx, kz) (c0 f0 + c1 f1 + s fB)
+        double r0[NE], r1[NE], rb[NE];
+        sp::load_rcol<NE>(a.PIr[0] + off, lane, ny, r0);
+        sp::load_rcol<NE>(a.PIr[1] + off, lane, ny, r1);
+        sp::load_rcol<NE>(a.PBr + off, lane, ny, rb);
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+        for (int e = 0; e < NE; ++e) P[e] = pc[0][e] + ((r0[e] * c0 + r1[e] * c1) + rb[e] * sp_other);
+        sp::load_rcol<NE>(a.VIv[0] + off, lane, ny, r0);
+        sp::load_rcol<NE>(a.VIv[1] + off, lane, ny, r1);
+        sp::load_rcol<NE>(a.VBv + off, lane, ny, rb);
 #pragma unroll
-            for (int c = 0; c < CH; ++c)
-                P[p][c] = pc[p][c] + ((r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[1 - p]);
-        load_rcol<CH>(a.VIv[0] + off, lane, ny, r0);
-        load_rcol<CH>(a.VIv[1] + off, lane, ny, r1);
-        load_rcol<CH>(a.VBv + off, lane, ny, rb);
+        for (int e = 0; e < NE; ++e) V[1][e] = vc[1][e] + ((r0[e] * c0 + r1[e] * c1) + rb[e] * sp_own);
+        sp::load_rcol<NE>(a.VIf[0] + off, lane, ny, r0);
+        sp::load_rcol<NE>(a.VIf[1] + off, lane, ny, r1);
+        sp::load_rcol<NE>(a.VBf + off, lane, ny, rb);
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c)
-                V[1][p][c] = vc[1][p][c] + ((r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[p]);
-        load_rcol<CH>(a.VIf[0] + off, lane, ny, r0);
-        load_rcol<CH>(a.VIf[1] + off, lane, ny, r1);
-        load_rcol<CH>(a.VBf + off, lane, ny, rb);
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const zd phi = (r0[p][c] * c0 + r1[p][c] * c1) + rb[p][c] * sig[1 - p];
-                V[0][p][c] = vc[0][p][c] + ikmul(kx, phi);
-                V[2][p][c] = vc[2][p][c] + ikmul(kz, phi);
-            }
+        for (int e = 0; e < NE; ++e) {
+            const zd phi = (r0[e] * c0 + r1[e] * c1) + rb[e] * sp_other;
+            V[0][e] = vc[0][e] + ikmul(kx, phi);
+            V[2][e] = vc[2][e] + ikmul(kz, phi);
+        }
     } else {
-    {
-        zd i0[2][CH], i1[2][CH], bb[2][CH];
-        if (a.correct) {
-            load_col<CH>(a.PI[0] + off, lane, ny, i0);
-            load_col<CH>(a.PI[1] + off, lane, ny, i1);
-        }
-        if (a.sigma) load_col<CH>(a.PB + off, lane, ny, bb);
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                zd v = pc[p][c];
-                if (a.correct) v = v + (cmul(c0, i0[p][c]) + cmul(c1, i1[p][c]));
-                if (a.sigma) v = v + cmul(sig[1 - p], bb[p][c]); // q = (j+1) mod 2 = 1 - p
-                P[p][c] = v;
+        {
+            zd i0[NE], i1[NE], bb[NE];
+            if (a.correct) {
+                sp::load_col<NE>(a.PI[0] + off, lane, ny, i0);
+                sp::load_col<NE>(a.PI[1] + off, lane, ny, i1);
             }
-    }
+            if (a.sigma) sp::load_col<NE>(a.PB + off, lane, ny, bb);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        zd i0[2][CH], i1[2][CH], bb[2][CH];
-        if (a.correct) {
-            load_col<CH>(a.VI[0][k] + off, lane, ny, i0);
-            load_col<CH>(a.VI[1][k] + off, lane, ny, i1);
-        }
-        if (a.sigma) load_col<CH>(a.VB[k] + off, lane, ny, bb);
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                zd v = vc[k][p][c];
-                if (a.correct) v = v + (cmul(c0, i0[p][c]) + cmul(c1, i1[p][c]));
-                if (a.sigma) v = v + cmul(sig[k == 1 ? p : 1 - p], bb[p][c]);
-                V[k][p][c] = v;
+            for (int e = 0; e < NE; ++e) {
+                zd v = pc[0][e];
+                if (a.correct) v = v + (cmul(c0, i0[e]) + cmul(c1, i1[e]));
+                if (a.sigma) v = v + cmul(sp_other, bb[e]);
+                P[e] = v;
             }
-    }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            zd i0[NE], i1[NE], bb[NE];
+            if (a.correct) {
+                sp::load_col<NE>(a.VI[0][k] + off, lane, ny, i0);
+                sp::load_col<NE>(a.VI[1][k] + off, lane, ny, i1);
+            }
+            if (a.sigma) sp::load_col<NE>(a.VB[k] + off, lane, ny, bb);
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                zd v = vc[k][e];
+                if (a.correct) v = v + (cmul(c0, i0[e]) + cmul(c1, i1[e]));
+                if (a.sigma) v = v + cmul(k == 1 ? sp_own : sp_other, bb[e]);
+                V[k][e] = v;
+            }
+        }
     }
 
     if (mode == MODE_STEP && zero) {
         // correct_zero_mode, then correct_flow on the mean mode
         const zd vmean = 0.5 * (vlo[1] + vup[1]);
-        double u_unit[2][CH], w_unit[2][CH];
-        load_rcol<CH>(a.u_unit, lane, ny, u_unit);
-        load_rcol<CH>(a.w_unit, lane, ny, w_unit);
-        double qx = 0.0, qz = 0.0;
+        double u_unit[NE], w_unit[NE];
+        sp::load_rcol<NE>(a.u_unit, lane, ny, u_unit);
+        sp::load_rcol<NE>(a.w_unit, lane, ny, w_unit);
+        double t[2] = {0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            V[1][0][c] = {0.0, 0.0};
-            V[1][1][c] = {0.0, 0.0};
-            const int j = jof<CH>(lane, c, 0);
-            if (j <= ny) {
-                qx -= V[0][0][c].x / (double)(j * j - 1);
-                qz -= V[2][0][c].x / (double)(j * j - 1);
+        for (int e = 0; e < NE; ++e) {
+            V[1][e] = {0.0, 0.0};
+            const int j = sp::jof<NE>(lane, e);
+            if (p == 0 && j <= ny) {
+                t[0] -= V[0][e].x / (double)(j * j - 1);
+                t[1] -= V[2][e].x / (double)(j * j - 1);
             }
         }
-        if (lane == 0) V[1][0][0] = vmean;
-        qx = wave_sum(qx);
-        qz = wave_sum(qz);
+        if (lane == 0) V[1][0] = vmean;
+        wave_sum<2>(t);
         double dpdx, dpdz;
         if (a.grad_pre == 1) {
             dpdx = a.grad_x;
             dpdz = a.grad_z;
         } else {
-            dpdx = (a.qx_mean - qx) / a.q_unit[0];
-            dpdz = (a.qz_mean - qz) / a.q_unit[1];
+            dpdx = (a.qx_mean - t[0]) / a.q_unit[0];
+            dpdz = (a.qz_mean - t[1]) / a.q_unit[1];
         }
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                V[0][p][c].x += dpdx * u_unit[p][c];
-                V[2][p][c].x += dpdz * w_unit[p][c];
-            }
+        for (int e = 0; e < NE; ++e) {
+            V[0][e].x += dpdx * u_unit[e];
+            V[2][e].x += dpdz * w_unit[e];
+        }
         if (lane == 0) {
             a.flow[0] = dpdx;
             a.flow[1] = dpdz;
         }
     }
-    if (mode == MODE_BPROBLEM) {
+    if constexpr (mode == MODE_BPROBLEM) {
         // sigma denominators (eqsolve_pv.cpp:466-476)
-        const zd vn = col_at<CH>(V[1], ny), vm = col_at<CH>(V[1], ny - 1), pn = col_at<CH>(P, ny);
+        const zd vn = sp::col_at<NE>(V[1], ny), vm = sp::col_at<NE>(V[1], ny - 1), pn = sp::col_at<NE>(P, ny);
         if (lane == 0) {
             zd s0 = {a.Re + wn_v * vn.x, wn_v * vn.y};
             zd s1 = zd{a.Re + wn_v * vm.x, wn_v * vm.y} - (a.Re * (2.0 * ny)) * pn;
@@ -443,15 +413,26 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
             a.sig_out[(size_t)q * 2 + 1] = s1;
         }
     }
-    store_col<CH>(a.P + off, lane, ny, P);
+    sp::store_col<NE>(a.P + off, lane, ny, P);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) store_col<CH>(a.V[k] + off, lane, ny, V[k]);
+    for (int k = 0; k < 3; ++k) sp::store_col<NE>(a.V[k] + off, lane, ny, V[k]);
 }
 
-void launch_solve(const SolveArgs &a, int E, hipStream_t st) {
+template <int NE> static void launch_solve_mode(const SolveArgs &a, hipStream_t st) {
     dim3 grid((a.NQ + 3) / 4), block(256);
-    if (E == 2) hipLaunchKernelGGL((k_solve<1>), grid, block, 0, st, a);
-    else if (E == 4) hipLaunchKernelGGL((k_solve<2>), grid, block, 0, st, a);
+    switch (a.mode) {
+    case MODE_STEP: hipLaunchKernelGGL((k_solve<NE, MODE_STEP>), grid, block, 0, st, a); break;
+    case MODE_GIVEN: hipLaunchKernelGGL((k_solve<NE, MODE_GIVEN>), grid, block, 0, st, a); break;
+    case MODE_INFLUENCE: hipLaunchKernelGGL((k_solve<NE, MODE_INFLUENCE>), grid, block, 0, st, a); break;
+    case MODE_BPROBLEM: hipLaunchKernelGGL((k_solve<NE, MODE_BPROBLEM>), grid, block, 0, st, a); break;
+    default: throw std::runtime_error("unknown solve mode");
+    }
+}
+// E = column stride / 64 complex entries per lane pair: two entries of one chain per lane for
+// columns up to 128, four up to 256
+void launch_solve(const SolveArgs &a, int E, hipStream_t st) {
+    if (E == 2) launch_solve_mode<2>(a, st);
+    else if (E == 4) launch_solve_mode<4>(a, st);
     else throw std::runtime_error("unsupported column length");
 }
 
@@ -535,53 +516,54 @@ void launch_normalize_influence(zd *ID0, zd *ID1, int NQ, int root_q, double aV1
 }
 
 // build_pressure_gradient_responses (eqsolve_pv.cpp:479-493) on the mean mode: one wavefront.
-template <int CH>
+template <int NE>
 __global__ void k_pg_response(const double *WAVX, const double *WAVZ, const double *WL_, const double *WD_,
                               const double *WR_, const double *ad_v, double g_v, double Re, double aV0, double bV0,
                               double aV2, double bV2, int ny, int JS, int root_q, double *u_unit, double *w_unit,
                               double *q_unit) {
-    const int lane = lane_id();
+    const int lane = lane_id(), p = sp::parity(lane);
     const size_t off = (size_t)root_q * JS;
-    double WL[2][CH], WD[2][CH], WR[2][CH], adv[2][CH];
-    load_rcol<CH>(WL_, lane, ny, WL);
-    load_rcol<CH>(WD_, lane, ny, WD);
-    load_rcol<CH>(WR_, lane, ny, WR);
-    load_rcol<CH>(ad_v + off, lane, ny, adv);
+    double WL[NE], WD[NE], WR[NE], adv[NE];
+    sp::load_rcol<NE>(WL_, lane, ny, WL);
+    sp::load_rcol<NE>(WD_, lane, ny, WD);
+    sp::load_rcol<NE>(WR_, lane, ny, WR);
+    sp::load_rcol<NE>(ad_v + off, lane, ny, adv);
     const double w = -(WAVX[0] * WAVX[0] + WAVZ[root_q] * WAVZ[root_q] + g_v);
-    zd rhs[2][CH];
+    zd rhs[2][NE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int e = 0; e < NE; ++e) rhs[0][e] = rhs[1][e] = zd{0.0, 0.0};
+    if (lane == 0) rhs[0][0] = rhs[1][0] = zd{Re, 0.0};
+    const double av[2] = {aV0, aV2}, bv[2] = {bV0, bV2};
+    const zd zw[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    zd sol[2][NE];
+    tau_solve<NE, 2>(rhs, sol, adv, WL, WD, WR, lane, ny, w, g_v, av, bv, true, zw, zw);
+    double qs[2] = {0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < CH; ++c) rhs[p][c] = {0.0, 0.0};
-    if (lane == 0) rhs[0][0] = {Re, 0.0};
     for (int comp = 0; comp < 2; ++comp) {
-        zd sol[2][CH];
-        tau_solve<CH>(rhs, sol, adv, WL, WD, WR, lane, ny, w, g_v, comp ? aV2 : aV0, comp ? bV2 : bV0, true,
-                      zd{0.0, 0.0}, zd{0.0, 0.0});
-        double qs = 0.0;
         double *unit = comp ? w_unit : u_unit;
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const int j = jof<CH>(lane, c, p);
-                if (j <= ny) {
-                    unit[j] = sol[p][c].x;
-                    if (p == 0) qs -= sol[p][c].x / (double)(j * j - 1);
-                }
+        for (int e = 0; e < NE; ++e) {
+            const int j = sp::jof<NE>(lane, e);
+            if (j <= ny) {
+                unit[j] = sol[comp][e].x;
+                if (p == 0) qs[comp] -= sol[comp][e].x / (double)(j * j - 1);
             }
-        qs = wave_sum(qs);
-        if (lane == 0) q_unit[comp] = qs;
+        }
+    }
+    wave_sum<2>(qs);
+    if (lane == 0) {
+        q_unit[0] = qs[0];
+        q_unit[1] = qs[1];
     }
 }
 void launch_pg_response(const double *WAVX, const double *WAVZ, const double *WL, const double *WD, const double *WR,
                         const double *ad_v, double g_v, double Re, const double *aV, const double *bV, int ny, int JS,
                         int root_q, double *u_unit, double *w_unit, double *q_unit, int E, hipStream_t st) {
     if (E == 2)
-        hipLaunchKernelGGL((k_pg_response<1>), dim3(1), dim3(64), 0, st, WAVX, WAVZ, WL, WD, WR, ad_v, g_v, Re, aV[0],
+        hipLaunchKernelGGL((k_pg_response<2>), dim3(1), dim3(64), 0, st, WAVX, WAVZ, WL, WD, WR, ad_v, g_v, Re, aV[0],
                            bV[0], aV[2], bV[2], ny, JS, root_q, u_unit, w_unit, q_unit);
     else
-        hipLaunchKernelGGL((k_pg_response<2>), dim3(1), dim3(64), 0, st, WAVX, WAVZ, WL, WD, WR, ad_v, g_v, Re, aV[0],
+        hipLaunchKernelGGL((k_pg_response<4>), dim3(1), dim3(64), 0, st, WAVX, WAVZ, WL, WD, WR, ad_v, g_v, Re, aV[0],
                            bV[0], aV[2], bV[2], ny, JS, root_q, u_unit, w_unit, q_unit);
 }
 
