@@ -86,6 +86,7 @@ struct cdns_ctx {
     int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
     int ymfma = 1; // y stages on the matrix cores (CDNS_YSTAGE=vector selects the vector-ALU kernels)
     int xpair = 1; // x-stage product FFT rows on lane pairs (CDNS_XPRODUCT=single keeps one lane per row)
+    int zpair = 1; // z-stage FFT rows on lane pairs (CDNS_ZSTAGE=single keeps one lane per row)
     std::vector<double> hWAVX, hWAVZ;
 
     // solver tables
@@ -230,6 +231,7 @@ ZArgs zargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs)
     a.KT = c->KT;
     a.NL = c->g.NL;
     a.rf = c->rf;
+    a.pair = c->zpair;
     return a;
 }
 XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs, int P) {
@@ -743,6 +745,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
         if (const char *e = getenv("CDNS_YSTAGE")) c->ymfma = std::string(e) != "vector";
         if (const char *e = getenv("CDNS_XPRODUCT")) c->xpair = std::string(e) != "single";
+        if (const char *e = getenv("CDNS_ZSTAGE")) c->zpair = std::string(e) != "single";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;
         g.nzk = 2 * g.NZS + 1;

@@ -29,6 +29,7 @@ struct ZArgs {
     int nkx, nzk, NZS, NQ, KT, NL;
     int rf; // use the register-resident FFT when available
     int nkxg; // register z forward: kx groups per (field, l)
+    int pair; // FFT rows on lane pairs (even factors)
 };
 
 // x stages (FFT over kx) and the cross product.

@@ -129,7 +129,7 @@ template <int N1, int N2> struct TwoPass {
 // output on threads t < N2 (arrays sized max(N1, N2)).  `tw` is the N-point root table
 // (exp(-2 pi i k / N)) in LDS, `T` the group's transpose buffer.  Sync is the caller's barrier.
 template <typename R, int N1, int N2, int S, int M, typename Sync>
-__device__ inline void two_pass(cpx<R> (&x)[M], int t, const cpx<R> *tw, cpx<R> *T, Sync sync) {
+__device__ __attribute__((always_inline)) inline void two_pass(cpx<R> (&x)[M], int t, const cpx<R> *tw, cpx<R> *T, Sync sync) {
     constexpr int N = N1 * N2, PA = N2 + 1;
     if (t < N1) {
         cpx<R> a[N2];
@@ -171,25 +171,45 @@ template <typename R> __device__ inline cpx<R> csel(bool c, cpx<R> a, cpx<R> b) 
     return {c ? a.x : b.x, c ? a.y : b.y};
 }
 
+// v (exp(S 2 pi i K / N) - 1), the twiddle step of the pair butterflies applied as a correction
+template <typename R, int K, int N, int S> __device__ inline cpx<R> twmul_m1(cpx<R> v) {
+    if constexpr ((4 * K) % N == 0) {
+        const cpx<R> w = twmul<R, K, N, S>(v);
+        return {w.x - v.x, w.y - v.y};
+    } else {
+        constexpr R c = (R)(Tw<K, N, S>::c - 1.0), s = (R)Tw<K, N, S>::s;
+        return {v.x * c - v.y * s, v.x * s + v.y * c};
+    }
+}
+
 // DIT: lane h holds x[2m + h], m < H; on return it holds X[H h + k], k < H.
+// Radix-2 across the pair: lane 1 twiddles its odd-half DFT (v + h v (W^K - 1)), then lane 0
+// forms E + W O = v + p and lane 1 E - W O = p - v, both as p + sg v with sg = 1 - 2h.
 template <typename R, int H, int S> __device__ inline void pair_dit(cpx<R> (&v)[H], int h) {
     RDft<R, H, S>::run(v);
+    const R hd = R(h), sg = R(1 - 2 * h);
     sfor<0, H>([&](auto k) {
         constexpr int K = decltype(k)::value;
-        v[K] = csel(h != 0, twmul<R, K, 2 * H, S>(v[K]), v[K]);
-    });
-    sfor<0, H>([&](auto k) {
-        constexpr int K = decltype(k)::value;
+        if constexpr (K > 0) {
+            const cpx<R> d = twmul_m1<R, K, 2 * H, S>(v[K]);
+            v[K] = {v[K].x + hd * d.x, v[K].y + hd * d.y};
+        }
         const cpx<R> p = pair_swap(v[K]);
-        v[K] = csel(h != 0, p - v[K], v[K] + p);
+        v[K] = {p.x + sg * v[K].x, p.y + sg * v[K].y};
     });
 }
 // DIF: lane h holds x[H h + m], m < H; on return it holds X[2k + h], k < H.
+// Lane 0 forms v + p, lane 1 (p - v) W^M: both p + sg v, then lane 1's twiddle as a correction.
 template <typename R, int H, int S> __device__ inline void pair_dif(cpx<R> (&v)[H], int h) {
+    const R hd = R(h), sg = R(1 - 2 * h);
     sfor<0, H>([&](auto m) {
         constexpr int M = decltype(m)::value;
         const cpx<R> p = pair_swap(v[M]);
-        v[M] = csel(h != 0, twmul<R, M, 2 * H, S>(p - v[M]), v[M] + p);
+        v[M] = {p.x + sg * v[M].x, p.y + sg * v[M].y};
+        if constexpr (M > 0) {
+            const cpx<R> d = twmul_m1<R, M, 2 * H, S>(v[M]);
+            v[M] = {v[M].x + hd * d.x, v[M].y + hd * d.y};
+        }
     });
     RDft<R, H, S>::run(v);
 }
@@ -199,7 +219,7 @@ template <typename R, int H, int S> __device__ inline void pair_dif(cpx<R> (&v)[
 // AIN_HALVES, line[t + N1 (H2 h + m)].  Pass B output: x[k] = X[t + N2 (H1 h + k)], which is the
 // halves input layout of the transposed plan.  x has max(N1, N2) / 2 entries.
 template <typename R, int N1, int N2, int S, bool AIN_HALVES, int M, typename Sync>
-__device__ inline void two_pass_pair(cpx<R> (&x)[M], int t, int h, const cpx<R> *tw, cpx<R> *T, Sync sync) {
+__device__ __attribute__((always_inline)) inline void two_pass_pair(cpx<R> (&x)[M], int t, int h, const cpx<R> *tw, cpx<R> *T, Sync sync) {
     constexpr int N = N1 * N2, PA = N2 + 1, H1 = N1 / 2, H2 = N2 / 2;
     static_assert(N1 % 2 == 0 && N2 % 2 == 0, "pair FFT needs even factors");
     if (t < N1) {

@@ -38,6 +38,20 @@ __device__ inline cpx<R> herm_pack(const cpx<R> *A, const cpx<R> *B, int k, int 
     return {av.x - bv.y, av.y + bv.x};
 }
 
+// herm_pack without branches: both loads are issued for every lane (index clamped to the
+// retained range) and the value is selected afterwards, so a wavefront's loads of several lines
+// can all be in flight at once.
+template <typename R>
+__device__ inline cpx<R> herm_pack_nb(const cpx<R> *A, const cpx<R> *B, int k, int N, int K, bool use) {
+    const bool mirror = k > K, zero = !use || (mirror && N - k > K);
+    int src = mirror ? N - k : k;
+    src = src < 0 ? 0 : (src > K ? K : src);
+    const cpx<R> av = A[src], bv = B[src];
+    const bool real_end = src == 0 || 2 * src == N;
+    const R ay = real_end ? R(0) : (mirror ? -av.y : av.y), by = real_end ? R(0) : (mirror ? -bv.y : bv.y);
+    return zero ? cpx<R>{R(0), R(0)} : cpx<R>{av.x - by, ay + bv.x};
+}
+
 template <typename R, int N>
 __device__ inline void load_roots(cpx<R> *tw, const void *src) {
     const cpx<R> *s = reinterpret_cast<const cpx<R> *>(src);
@@ -129,23 +143,30 @@ __global__ void __launch_bounds__(64) k_xprod_pair(XArgs a) {
     __shared__ cpx<R> tw[N];
     __shared__ cpx<R> buf[NG][LB];
     const int lane = threadIdx.x, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
-    load_roots<R, N>(tw, a.tw);
-    __syncthreads();
     const int np = a.NL * a.NZ2, K = a.nkx - 1;
     const int p = blockIdx.x * NG + g;
     const bool valid = p < np;
     cpx<R> *B = buf[g];
+    // each line's loads are branch-free, so they are all in flight before the first use; the
+    // first line's loads also overlap the root table's
     cpx<R> ph[3][MH];
+    const size_t prow = (size_t)(valid ? p : 0) * a.nkx;
     sfor<0, 3>([&](auto line) {
         constexpr int LN = decltype(line)::value;
-        cpx<R> x[MH];
-        const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]) + (size_t)p * a.nkx;
-        const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]) + (size_t)p * a.nkx;
-        sfor<0, MH>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
-        if (valid && t < N1)
-            sfor<0, H2>([&](auto m) { x[m] = herm_pack(A, Bv, t + N1 * (2 * decltype(m)::value + h), N, K); });
-        two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, B, wsync);
-        sfor<0, MH>([&](auto j) { ph[LN][j] = x[j]; });
+        const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]) + prow;
+        const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]) + prow;
+        sfor<0, MH>([&](auto m) {
+            constexpr int M = decltype(m)::value;
+            if constexpr (M < H2)
+                ph[LN][M] = herm_pack_nb(A, Bv, t + N1 * (2 * M + h), N, K, valid && t < N1);
+            else
+                ph[LN][M] = cpx<R>{R(0), R(0)};
+        });
+        if constexpr (LN == 0) {
+            load_roots<R, N>(tw, a.tw);
+            __syncthreads();
+        }
+        two_pass_pair<R, N1, N2, +1, false>(ph[LN], t, h, tw, B, wsync);
     });
     // u x w at the points n = t + N2 (H1 h + k) held by this lane
     cpx<R> F[MH], Fz[MH];
@@ -375,8 +396,6 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
     __shared__ cpx<R> tw[N];
     __shared__ cpx<R> buf[NG][LB];
     const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
-    load_roots<R, N>(tw, a.tw);
-    __syncthreads();
     // neighbouring kx groups of one (field, l) read the same cache lines: keep them on one XCD
     const int L = xcd_block((int)gridDim.x);
     const int kxg = L % a.nkxg, lf = L / a.nkxg, l = lf % a.NL, f = lf / a.NL;
@@ -388,6 +407,8 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
     cpx<R> x[MX];
     sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
     if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[(size_t)(t + N1 * decltype(n2)::value) * a.nkx]; });
+    load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
+    __syncthreads();
     two_pass<R, N1, N2, -1>(x, t, tw, buf[g], wsync);
     if (valid && t < N2) {
         const R inv = R(1.0 / N);
@@ -398,6 +419,67 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
             if (s <= NZS) Y[s + NZS] = inv * x[k1];
             if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
         });
+    }
+}
+
+// -------------------------------------------------------- z backward, lane pairs --
+// The z backward above with each FFT row on a lane pair (radix-2 across the pair): half the data
+// registers per lane, two lines per wavefront, twice the waves per CU.  (For the forward stage the
+// one-lane rows stay faster: with two lines per wavefront each row read is only 2 kx wide.)
+template <typename R, int N1, int N2> struct ZTileP {
+    using Gm = PGeo<N1, N2>;
+    static constexpr int N = Gm::N;
+    static constexpr int lds_of(int kb) {
+        return (kb * Gm::LB > N * (kb + 1) ? kb * Gm::LB : N * (kb + 1)) + N; // tile + roots
+    }
+    static constexpr int KB = lds_of(16) * (int)sizeof(cpx<R>) <= 65536 ? 16 : 8;
+    static constexpr int WAVES = KB / Gm::NG, PITCH = KB + 1;
+    static constexpr int LDS = lds_of(KB) - N;
+};
+template <typename R, int N1, int N2> constexpr int kZPThreads = 64 * ZTileP<R, N1, N2>::WAVES;
+
+template <typename R, int N1, int N2>
+__global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) {
+    using Gm = PGeo<N1, N2>;
+    using T = ZTileP<R, N1, N2>;
+    constexpr int kZLines = T::KB;
+    constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, LB = Gm::LB, PITCH = T::PITCH;
+    constexpr int H1 = N1 / 2, H2 = N2 / 2;
+    __shared__ cpx<R> tw[N];
+    __shared__ cpx<R> lds[T::LDS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
+    const int line = wv * NG + g;
+    const int kx0 = blockIdx.x * kZLines, kx = kx0 + line, l = blockIdx.y, f = blockIdx.z;
+    const int nl = min(kZLines, a.nkx - kx0);
+    const bool valid = line < nl;
+    const int NZS = a.NZS;
+    const cpx<R> *Y = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * a.NQ +
+                      (size_t)(valid ? kx : 0) * a.nzk;
+    cpx<R> x[MH];
+    sfor<0, MH>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
+    if (valid && t < N1) {
+        sfor<0, H2>([&](auto m) {
+            const int s = t + N1 * (2 * decltype(m)::value + h);
+            // slot s holds kz = s (s <= NZS) or s - N (s >= N - NZS); at an overlap the slot
+            // N - NZS takes kz = -NZS (transforms.cpp:717-726)
+            int kz = 1 << 30;
+            if (s <= NZS) kz = s;
+            else if (s >= N - NZS) kz = s - N;
+            if (s == N - NZS && s <= NZS) kz = -NZS;
+            if (kz != (1 << 30)) x[m] = Y[kz + NZS];
+        });
+    }
+    load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
+    __syncthreads();
+    two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, lds + line * LB, wsync);
+    // two_pass_pair ends with a barrier: the tile may now overwrite the transpose buffers
+    if (valid && t < N2)
+        sfor<0, H1>([&](auto k) { lds[(t + N2 * (H1 * h + decltype(k)::value)) * PITCH + line] = x[k]; });
+    __syncthreads();
+    cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx0;
+    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
+        const int iz = e / nl, li = e - iz * nl;
+        Z[(size_t)iz * a.nkx + li] = lds[iz * PITCH + li];
     }
 }
 
@@ -425,6 +507,14 @@ template <typename R, int N1, int N2> static void go_xfwd(const XArgs &a, hipStr
                        st, a);
 }
 template <typename R, int N1, int N2> static void go_zbwd(const ZArgs &a, int nf, hipStream_t st) {
+    if constexpr (N1 % 2 == 0 && N2 % 2 == 0) {
+        if (a.pair) {
+            using T = ZTileP<R, N1, N2>;
+            hipLaunchKernelGGL((k_zbwd_pair<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf),
+                               dim3(64 * T::WAVES), 0, st, a);
+            return;
+        }
+    }
     using T = ZTile<R, N1, N2>;
     hipLaunchKernelGGL((k_zbwd_rf<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES), 0,
                        st, a);

@@ -27,7 +27,7 @@ FFT_KINDS = ["register", "stockham", "single-lane-rows"]
 
 
 def _use_fft(monkeypatch, kind):
-    for v in ("CDNS_FFT", "CDNS_YSTAGE", "CDNS_XPRODUCT", "CDNS_SOLVER_TABLES"):
+    for v in ("CDNS_FFT", "CDNS_YSTAGE", "CDNS_XPRODUCT", "CDNS_ZSTAGE", "CDNS_SOLVER_TABLES"):
         monkeypatch.delenv(v, raising=False)
     if kind == "stockham":
         monkeypatch.setenv("CDNS_FFT", "stockham")
@@ -35,6 +35,7 @@ def _use_fft(monkeypatch, kind):
         monkeypatch.setenv("CDNS_SOLVER_TABLES", "complex")
     elif kind == "single-lane-rows":
         monkeypatch.setenv("CDNS_XPRODUCT", "single")
+        monkeypatch.setenv("CDNS_ZSTAGE", "single")
 
 
 @pytest.mark.parametrize("fft", FFT_KINDS)
