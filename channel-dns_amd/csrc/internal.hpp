@@ -32,6 +32,14 @@ struct ZArgs {
     int pair; // FFT rows on lane pairs (even factors)
 };
 
+// Layout of the z-physical fields produced by the x forward stages and read by the z forward:
+// line-major [l][kx][iz] (each z line contiguous), point p = l NZ2 + iz.  The z backward writes
+// the row-major [l][iz][kx] that the x backward stages read.
+__host__ __device__ inline size_t zline_index(int p, int kx, int nkx, int NZ2) {
+    const int l = p / NZ2, iz = p - l * NZ2;
+    return ((size_t)l * nkx + kx) * NZ2 + iz;
+}
+
 // x stages (FFT over kx) and the cross product.
 struct XArgs {
     const void *in;

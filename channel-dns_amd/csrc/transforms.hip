@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_zbwd(ZArgs a) {
     }
 }
 
-// forward: Z[f][l][iz][kx] -> FFT(-) -> Y[f][l][kx*nzk + kzi] = X(slot)/NZ2
+// forward: Z[f][l][kx][iz] (line-major) -> FFT(-) -> Y[f][l][kx*nzk + kzi] = X(slot)/NZ2
 template <typename R, int IT>
 __global__ void __launch_bounds__(256) k_zfwd(ZArgs a) {
     extern __shared__ unsigned char smem_raw[];
@@ -203,11 +203,10 @@ __global__ void __launch_bounds__(256) k_zfwd(ZArgs a) {
     const int kt0 = blockIdx.x * a.KT, l = blockIdx.y, f = blockIdx.z;
     const int nl = min(a.KT, a.nkx - kt0);
     const int N = a.plan.n, NZS = a.NZS;
-    const cpx<R> *__restrict__ Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx;
-    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-        const int iz = e / nl, ln = e - iz * nl;
-        buf[ln * N + iz] = Z[(size_t)iz * a.nkx + kt0 + ln];
-    }
+    // line-major input [l][kx][iz]: the block's lines are one contiguous run
+    const cpx<R> *__restrict__ Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs +
+                                   ((size_t)l * a.nkx + kt0) * N;
+    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) buf[e] = Z[e];
     __syncthreads();
     fft_block<R, IT>(buf, nl, a.plan, reinterpret_cast<const cpx<R> *>(a.tw), -1);
     const R inv = R(1.0 / N);
@@ -327,8 +326,8 @@ __global__ void __launch_bounds__(256) k_xprod(XArgs a) {
         if (sel < 2) { pa = pb = 2 * h + sel; ca = 0; cb = 1; }
         else { pa = 2 * h; pb = 2 * h + 1; ca = cb = 2; }
         const int pta = p0 + pa, ptb = p0 + pb;
-        if (pta < np) reinterpret_cast<cpx<R> *>(a.fout[ca])[(size_t)pta * a.nkx + k] = first;
-        if (ptb < np) reinterpret_cast<cpx<R> *>(a.fout[cb])[(size_t)ptb * a.nkx + k] = second;
+        if (pta < np) reinterpret_cast<cpx<R> *>(a.fout[ca])[zline_index(pta, k, a.nkx, a.NZ2)] = first;
+        if (ptb < np) reinterpret_cast<cpx<R> *>(a.fout[cb])[zline_index(ptb, k, a.nkx, a.NZ2)] = second;
     }
 }
 
@@ -398,8 +397,8 @@ __global__ void __launch_bounds__(256) k_xfwd_phys(XArgs a) {
         const cpx<R> F = buf[h * N + k], G = conjc(buf[h * N + (k == 0 ? 0 : N - k)]);
         const cpx<R> s = F + G, d = F - G;
         const int pa = p0 + 2 * h, pb = pa + 1;
-        if (pa < np) ZN[(size_t)pa * a.nkx + k] = {half * s.x, half * s.y};
-        if (pb < np) ZN[(size_t)pb * a.nkx + k] = {half * d.y, -half * d.x};
+        if (pa < np) ZN[zline_index(pa, k, a.nkx, a.NZ2)] = {half * s.x, half * s.y};
+        if (pb < np) ZN[zline_index(pb, k, a.nkx, a.NZ2)] = {half * d.y, -half * d.x};
     }
 }
 

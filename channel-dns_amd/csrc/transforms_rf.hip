@@ -20,6 +20,14 @@ template <int N1, int N2> struct Geo {
 };
 
 __device__ inline void wsync() { __syncthreads(); }
+// Ordering of one wavefront's LDS accesses: a lane group's transpose buffer belongs to one
+// wavefront, whose LDS instructions execute in program order, so the FFT passes need only a
+// compiler-level fence, not a workgroup barrier that would make the other wavefronts wait.
+__device__ inline void lsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename R>
 __device__ inline cpx<R> herm_pack(const cpx<R> *A, const cpx<R> *B, int k, int N, int K) {
@@ -108,8 +116,8 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         for (int k = t; k <= K; k += G) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm, d = f - gm;
-            Nx[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
-            Ny[(size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
+            Nx[zline_index(p, k, a.nkx, a.NZ2)] = {half * s.x, half * s.y};
+            Ny[zline_index(p, k, a.nkx, a.NZ2)] = {half * d.y, -half * d.x};
         }
     wsync();
     two_pass<R, N2, N1, -1>(Fz, t, tw, B, wsync);
@@ -119,7 +127,7 @@ __global__ void __launch_bounds__(64) k_xprod_rf(XArgs a) {
         for (int k = t; k <= K; k += G) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm;
-            Nz[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            Nz[zline_index(p, k, a.nkx, a.NZ2)] = {half * s.x, half * s.y};
         }
 }
 
@@ -136,17 +144,24 @@ template <int N1, int N2> struct PGeo {
     static constexpr int LB = Geo<N1, N2>::LB;
 };
 
+// points per block: 8 in FP64, 16 in FP32, so that each output row segment [l][kx][iz..] is 128 B
+template <typename R> constexpr int kXWaves = sizeof(R) == 8 ? 4 : 8;
+
 template <typename R, int N1, int N2>
-__global__ void __launch_bounds__(64) k_xprod_pair(XArgs a) {
+__global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     using Gm = PGeo<N1, N2>;
+    constexpr int W = kXWaves<R>;
     constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, LB = Gm::LB, H1 = N1 / 2, H2 = N2 / 2;
+    constexpr int PB = W * NG; // points per block
+    constexpr int KP = (128 + G - 1) / G;
     __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> buf[NG][LB];
-    const int lane = threadIdx.x, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
+    __shared__ cpx<R> buf[PB][LB];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
+    const int gb = wv * NG + g;
     const int np = a.NL * a.NZ2, K = a.nkx - 1;
-    const int p = blockIdx.x * NG + g;
+    const int p0 = blockIdx.x * PB, p = p0 + gb;
     const bool valid = p < np;
-    cpx<R> *B = buf[g];
+    cpx<R> *B = buf[gb];
     // each line's loads are branch-free, so they are all in flight before the first use; the
     // first line's loads also overlap the root table's
     cpx<R> ph[3][MH];
@@ -166,7 +181,7 @@ __global__ void __launch_bounds__(64) k_xprod_pair(XArgs a) {
             load_roots<R, N>(tw, a.tw);
             __syncthreads();
         }
-        two_pass_pair<R, N1, N2, +1, false>(ph[LN], t, h, tw, B, wsync);
+        two_pass_pair<R, N1, N2, +1, false>(ph[LN], t, h, tw, B, lsync);
     });
     // u x w at the points n = t + N2 (H1 h + k) held by this lane
     cpx<R> F[MH], Fz[MH];
@@ -182,29 +197,59 @@ __global__ void __launch_bounds__(64) k_xprod_pair(XArgs a) {
         });
     }
     const R half = R(0.5) / R(N);
-    cpx<R> *Nx = reinterpret_cast<cpx<R> *>(a.fout[0]), *Ny = reinterpret_cast<cpx<R> *>(a.fout[1]),
-           *Nz = reinterpret_cast<cpx<R> *>(a.fout[2]);
-    // forward of (Nx + i Ny): transposed plan, halves input layout already in place
-    two_pass_pair<R, N2, N1, -1, true>(F, t, h, tw, B, wsync);
+    // the block's retained coefficients go out as rows [l][kx][iz0 .. iz0 + PB) of the line-major
+    // layout: B[k] and B[nkx + k] of each point, gathered after a block barrier
+    // thread i writes point j = i mod PB of rows i / PB, i / PB + 64 W / PB, ...
+    const int wj = threadIdx.x % PB, wr = threadIdx.x / PB, pw = p0 + wj;
+    const int wl = pw / a.NZ2;
+    const size_t wbase = ((size_t)wl * a.nkx) * a.NZ2 + (pw - wl * a.NZ2);
+    auto write_rows = [&](int nfo, int fo0) {
+        __syncthreads();
+        if (pw < np)
+            for (int fo = 0; fo < nfo; ++fo) {
+                cpx<R> *out = reinterpret_cast<cpx<R> *>(a.fout[fo0 + fo]) + wbase;
+                for (int k = wr; k < a.nkx; k += 64 * W / PB) out[(size_t)k * a.NZ2] = buf[wj][fo * a.nkx + k];
+            }
+        __syncthreads();
+    };
+    // forward of (Nx + i Ny): transposed plan, halves input layout already in place; the separated
+    // coefficients wait in registers while the Nz transform reuses the buffer
+    two_pass_pair<R, N2, N1, -1, true>(F, t, h, tw, B, lsync);
     if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = F[k]; });
-    wsync();
-    if (valid)
-        for (int k = r; k <= K; k += G) {
+    lsync();
+    cpx<R> sx[KP], sy[KP], sz[KP];
+    sfor<0, KP>([&](auto i) {
+        const int k = r + G * decltype(i)::value;
+        if (k <= K) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm, d = f - gm;
-            Nx[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
-            Ny[(size_t)p * a.nkx + k] = {half * d.y, -half * d.x};
+            sx[i] = {half * s.x, half * s.y};
+            sy[i] = {half * d.y, -half * d.x};
         }
-    wsync();
-    two_pass_pair<R, N2, N1, -1, true>(Fz, t, h, tw, B, wsync);
+    });
+    lsync();
+    two_pass_pair<R, N2, N1, -1, true>(Fz, t, h, tw, B, lsync);
     if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = Fz[k]; });
-    wsync();
-    if (valid)
-        for (int k = r; k <= K; k += G) {
+    lsync();
+    sfor<0, KP>([&](auto i) {
+        const int k = r + G * decltype(i)::value;
+        if (k <= K) {
             const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm;
-            Nz[(size_t)p * a.nkx + k] = {half * s.x, half * s.y};
+            sz[i] = {half * s.x, half * s.y};
         }
+    });
+    // all of the group's reads precede its writes (one wavefront, LDS in program order)
+    lsync();
+    sfor<0, KP>([&](auto i) {
+        const int k = r + G * decltype(i)::value;
+        if (k <= K) {
+            B[k] = sx[i];
+            B[a.nkx + k] = sy[i];
+            B[2 * a.nkx + k] = sz[i];
+        }
+    });
+    write_rows(3, 0);
 }
 
 // ------------------------------------------- single-field x stages (physical) --
@@ -281,17 +326,16 @@ __global__ void __launch_bounds__(64) k_xfwd_phys_rf(XArgs a) {
     for (int k = t; k <= K; k += G) {
         const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
         const cpx<R> s = f + gm, d = f - gm;
-        if (pa < np) ZN[(size_t)pa * a.nkx + k] = {half * s.x, half * s.y};
-        if (pb < np) ZN[(size_t)pb * a.nkx + k] = {half * d.y, -half * d.x};
+        if (pa < np) ZN[zline_index(pa, k, a.nkx, a.NZ2)] = {half * s.x, half * s.y};
+        if (pb < np) ZN[zline_index(pb, k, a.nkx, a.NZ2)] = {half * d.y, -half * d.x};
     }
 }
 
 // ------------------------------------------------------------------ z stages --
 // Backward: a block owns KB (16, or 8 for the largest lengths) consecutive kx lines of one
 // (field, l), one lane group per line; its output (Z[l][iz][kx], kx contiguous) goes through an LDS
-// tile [iz][kx] so that HBM sees whole KB-line rows (256 B in FP64).  Forward: in FP64 one
-// wavefront per block with lines read straight from HBM (a staged 50 KB tile serialises the loads
-// before the FFT at two blocks per CU); in FP32 the tiled form below.
+// tile [iz][kx] so that HBM sees whole KB-line rows (256 B in FP64).  Forward: the x stages write
+// the line-major layout [l][kx][iz], so one wavefront reads its lines as contiguous runs.
 template <typename R, int N1, int N2> struct ZTile {
     using Gm = Geo<N1, N2>;
     static constexpr int N = Gm::N;
@@ -349,45 +393,6 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
     }
 }
 
-// forward, tiled: the block's KB lines are read as whole [iz][kx] rows into LDS first (pays off
-// in FP32, where the tile is small and several blocks share a CU).
-template <typename R, int N1, int N2>
-__global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zfwd_tile(ZArgs a) {
-    using Gm = Geo<N1, N2>;
-    using T = ZTile<R, N1, N2>;
-    constexpr int kZLines = T::KB;
-    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB, PITCH = T::PITCH;
-    __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> lds[T::LDS];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
-    const int line = wv * NG + g;
-    load_roots<R, N>(tw, a.tw);
-    const int kx0 = blockIdx.x * kZLines, kx = kx0 + line, l = blockIdx.y, f = blockIdx.z;
-    const int nl = min(kZLines, a.nkx - kx0);
-    const bool valid = line < nl;
-    const int NZS = a.NZS;
-    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx + kx0;
-    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
-        const int iz = e / nl, li = e - iz * nl;
-        lds[iz * PITCH + li] = Z[(size_t)iz * a.nkx + li];
-    }
-    __syncthreads();
-    cpx<R> x[MX];
-    sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
-    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = lds[(t + N1 * decltype(n2)::value) * PITCH + line]; });
-    __syncthreads(); // the transpose buffers below reuse the tile
-    two_pass<R, N1, N2, -1>(x, t, tw, lds + line * LB, wsync);
-    if (valid && t < N2) {
-        const R inv = R(1.0 / N);
-        cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
-        sfor<0, N1>([&](auto k1) {
-            const int s = t + N2 * decltype(k1)::value;
-            if (s <= NZS) Y[s + NZS] = inv * x[k1];
-            if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
-        });
-    }
-}
-
 // forward: line (f, l, kx) over iz; keep the retained kz, x 1/NZ2.
 template <typename R, int N1, int N2>
 __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
@@ -402,14 +407,15 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
     const int kx = kxg * NG + g;
     const bool valid = kx < a.nkx;
     const int NZS = a.NZS;
-    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs + (size_t)l * N * a.nkx +
-                      (valid ? kx : 0);
+    // line (f, l, kx) of the line-major layout: N contiguous values
+    const cpx<R> *Z = reinterpret_cast<const cpx<R> *>(a.in) + (size_t)f * a.in_fs +
+                      ((size_t)l * a.nkx + (valid ? kx : 0)) * N;
     cpx<R> x[MX];
     sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
-    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[(size_t)(t + N1 * decltype(n2)::value) * a.nkx]; });
+    if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[t + N1 * decltype(n2)::value]; });
     load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
     __syncthreads();
-    two_pass<R, N1, N2, -1>(x, t, tw, buf[g], wsync);
+    two_pass<R, N1, N2, -1>(x, t, tw, buf[g], lsync);
     if (valid && t < N2) {
         const R inv = R(1.0 / N);
         cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
@@ -471,8 +477,8 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
     }
     load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
     __syncthreads();
-    two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, lds + line * LB, wsync);
-    // two_pass_pair ends with a barrier: the tile may now overwrite the transpose buffers
+    two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, lds + line * LB, lsync);
+    __syncthreads(); // every line's transposes are done: the tile may overwrite their buffers
     if (valid && t < N2)
         sfor<0, H1>([&](auto k) { lds[(t + N2 * (H1 * h + decltype(k)::value)) * PITCH + line] = x[k]; });
     __syncthreads();
@@ -486,10 +492,11 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
 // ------------------------------------------------------------------ launch --
 template <typename R, int N1, int N2> static void go_xprod(const XArgs &a, hipStream_t st) {
     if constexpr (N1 % 2 == 0 && N2 % 2 == 0) {
-        if (a.pair) {
-            using Gm = PGeo<N1, N2>;
-            hipLaunchKernelGGL((k_xprod_pair<R, N1, N2>), dim3((a.NL * a.NZ2 + Gm::NG - 1) / Gm::NG), dim3(64), 0, st,
-                               a);
+        // the three retained rows of a point are staged in its transpose buffer
+        if (a.pair && 3 * a.nkx <= PGeo<N1, N2>::LB) {
+            constexpr int PB = kXWaves<R> * PGeo<N1, N2>::NG;
+            hipLaunchKernelGGL((k_xprod_pair<R, N1, N2>), dim3((a.NL * a.NZ2 + PB - 1) / PB), dim3(64 * kXWaves<R>),
+                               0, st, a);
             return;
         }
     }
@@ -520,16 +527,10 @@ template <typename R, int N1, int N2> static void go_zbwd(const ZArgs &a, int nf
                        st, a);
 }
 template <typename R, int N1, int N2> static void go_zfwd(const ZArgs &a, int nf, hipStream_t st) {
-    if constexpr (sizeof(R) == 4) {
-        using T = ZTile<R, N1, N2>;
-        hipLaunchKernelGGL((k_zfwd_tile<R, N1, N2>), dim3((a.nkx + T::KB - 1) / T::KB, a.NL, nf), dim3(64 * T::WAVES),
-                           0, st, a);
-    } else {
-        using Gm = Geo<N1, N2>;
-        ZArgs b = a;
-        b.nkxg = (a.nkx + Gm::NG - 1) / Gm::NG;
-        hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3(b.nkxg * a.NL * nf), dim3(64), 0, st, b);
-    }
+    using Gm = Geo<N1, N2>;
+    ZArgs b = a;
+    b.nkxg = (a.nkx + Gm::NG - 1) / Gm::NG;
+    hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3(b.nkxg * a.NL * nf), dim3(64), 0, st, b);
 }
 
 // Supported lengths: N = N1 * N2 with N1, N2 <= 16.
