@@ -5,7 +5,7 @@ import sys
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["api.hip", "transforms.hip", "transforms_rf.hip", "transforms_y.hip", "spectral.hip", "solver.hip", "adjoint.hip"]
+SOURCES = ["api.hip", "transforms.hip", "transforms_rf.hip", "transforms_y.hip", "spectral.hip", "solver.hip", "adjoint.hip", "diagnostics.hip"]
 OUT = os.path.join(HERE, "libcdns_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 

@@ -2,6 +2,10 @@
 
 The classes mirror the reference's interfaces for this path:
   Transforms.channel_xyzfft2      src/fft/transforms.hpp:40-43
+  Transforms.channel_xyzfft       src/fft/transforms.hpp:39 (coarse grid)
+  FieldIO.to_physical             src/io/field_io.cpp:199-220
+  Statistics.reduce               src/sim/statistics.cpp:54-109
+  Diagnostics.evaluate            src/sim/diagnostics.cpp:39-75
   vector_ops.curl / cross_product src/solver/vector_ops.hpp:22,33
   TimeStepper.nonlinear_term      src/solver/timestepper.hpp:10-11
   EqSolvePV.setup / solve_modes   src/solver/eqsolve_pv.hpp:17-51
@@ -28,7 +32,8 @@ ABI_FUNCTIONS = [
     "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup", "cdns_solve_modes", "cdns_get_table",
     "cdns_set_state", "cdns_get_state", "cdns_step", "cdns_run", "cdns_flow_gradients", "cdns_time",
     "cdns_profile", "cdns_profile_read", "cdns_profile_reset", "cdns_set_body_force", "cdns_objective",
-    "cdns_adjoint_step", "cdns_adjoint_run",
+    "cdns_adjoint_step", "cdns_adjoint_run", "cdns_xyzfft", "cdns_to_physical", "cdns_plane_moments",
+    "cdns_diagnostics",
 ]
 
 
@@ -78,10 +83,15 @@ def lib(path=None):
         L.cdns_objective.argtypes = [vp, i, vp, vp, vp, i]
         L.cdns_adjoint_step.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, i]
         L.cdns_adjoint_run.argtypes = [vp, vp, vp, i, i, i, vp, vp, vp, i]
+        L.cdns_xyzfft.argtypes = [vp, ctypes.c_char, vp, vp, i, i]
+        L.cdns_to_physical.argtypes = [vp, vp, vp, vp, i]
+        L.cdns_plane_moments.argtypes = [vp, i, vp, vp]
+        L.cdns_diagnostics.argtypes = [vp, vp, vp]
         for n in ("cdns_xyzfft2", "cdns_curl", "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup",
                   "cdns_solve_modes", "cdns_get_table", "cdns_set_state", "cdns_get_state", "cdns_step",
                   "cdns_run", "cdns_flow_gradients", "cdns_profile", "cdns_profile_read", "cdns_profile_reset",
-                  "cdns_set_body_force", "cdns_objective", "cdns_adjoint_step", "cdns_adjoint_run"):
+                  "cdns_set_body_force", "cdns_objective", "cdns_adjoint_step", "cdns_adjoint_run", "cdns_xyzfft",
+                  "cdns_to_physical", "cdns_plane_moments", "cdns_diagnostics"):
             getattr(L, n).restype = i
         _LIB = L
     return _LIB
@@ -172,6 +182,43 @@ class Channel:
         else:
             sp = ctypes.c_void_p(spec.data_ptr() + 16 * offset)
         self._check(lib().cdns_xyzfft2(self._h, direction.encode(), _ptr(phys), sp, int(stride), where))
+
+    def coarse_phys(self):
+        m = self.cfg
+        return np.zeros((m["NZ"], m["NY"] + 1, m["NX"]), np.float64)
+
+    # Transforms::channel_xyzfft (coarse grid)
+    def xyzfft(self, direction, phys, spec, stride=1, offset=0):
+        where = _where(phys)
+        if where == HOST:
+            sp = ctypes.c_void_p(spec.ctypes.data + 16 * offset)
+        else:
+            sp = ctypes.c_void_p(spec.data_ptr() + 16 * offset)
+        self._check(lib().cdns_xyzfft(self._h, direction.encode(), _ptr(phys), sp, int(stride), where))
+
+    # FieldIO::to_physical: (velocity [3][NZ][NY+1][NX], pressure [NZ][NY+1][NX], finite)
+    def to_physical(self):
+        m = self.cfg
+        vel = np.zeros((3, m["NZ"], m["NY"] + 1, m["NX"]), np.float64)
+        pr = self.coarse_phys()
+        fin = ctypes.c_int(0)
+        self._check(lib().cdns_to_physical(self._h, _ptr(vel), _ptr(pr), ctypes.byref(fin), HOST))
+        return vel, pr, bool(fin.value)
+
+    # Statistics::reduce on the state: (sums [NY+1][20], finite [NY+1])
+    def plane_moments(self, higher=True):
+        n = self.cfg["NY"] + 1
+        sums = np.zeros((n, 20), np.float64)
+        fin = np.zeros(n, np.int32)
+        self._check(lib().cdns_plane_moments(self._h, int(bool(higher)), _ptr(sums), _ptr(fin)))
+        return sums, fin.astype(bool)
+
+    # Diagnostics::evaluate on the state: {"ekt", "tke", "divergence_l2", "finite"}
+    def diagnostics(self):
+        out = np.zeros(3, np.float64)
+        fin = ctypes.c_int(0)
+        self._check(lib().cdns_diagnostics(self._h, _ptr(out), ctypes.byref(fin)))
+        return {"ekt": float(out[0]), "tke": float(out[1]), "divergence_l2": float(out[2]), "finite": bool(fin.value)}
 
     # vector_ops
     def curl(self, v):

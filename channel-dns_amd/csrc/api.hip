@@ -62,6 +62,30 @@ std::vector<long double> cospi_table(int N) {
     return t;
 }
 
+// y-stage matrices for N + 1 Gauss-Lobatto points from exact-angle cosines: synthesis B[j][l]
+// (l <= N/2) and the DCT-I analysis weights split by parity, Fe[l][j/2] (j even), Fo[l][j/2] (odd).
+void y_tables(int N, int nj, int NLH, std::vector<double> &B, std::vector<double> &Fe, std::vector<double> &Fo) {
+    const auto cs = cospi_table(N);
+    B.assign((size_t)nj * NLH, 0.0);
+    for (int j = 0; j < nj; ++j)
+        for (int l = 0; l < NLH; ++l) {
+            const long double bw = (j == N) ? 0.5L : 1.0L; // REDFT00 endpoint weight x the 0.5 z factor
+            B[(size_t)j * NLH + l] = (double)(bw * cs[((long long)j * l) % (2 * N)]);
+        }
+    const int nje = (nj + 1) / 2, njo = nj / 2;
+    Fe.assign((size_t)NLH * nje, 0.0);
+    Fo.assign((size_t)NLH * std::max(njo, 1), 0.0);
+    for (int l = 0; l < NLH; ++l) {
+        const long double wl = (l == 0) ? 1.0L : 2.0L;
+        for (int j = 0; j < nj; ++j) {
+            const long double sc = j == 0 ? 1.0L / (2.0L * N) : 1.0L / N;
+            const double v = (double)(sc * wl * cs[((long long)j * l) % (2 * N)]);
+            if (j % 2 == 0) Fe[(size_t)l * nje + j / 2] = v;
+            else Fo[(size_t)l * njo + j / 2] = v;
+        }
+    }
+}
+
 struct Event2 {
     hipEvent_t a, b;
     int name;
@@ -116,6 +140,15 @@ struct cdns_ctx {
     zd *a_lNs[3] = {}, *a_lw[3] = {}, *a_lUs[3] = {}, *a_lWs[3] = {};
     void *ZL = nullptr, *ZNa = nullptr;
     double *Jpart = nullptr;
+
+    // coarse grid NX x (NY+1) x NZ (Transforms::channel_xyzfft): FP64 tables and plans, built on
+    // first use; the transforms run in the product-grid workspaces, which are larger
+    bool coarse_ready = false;
+    double *cy_b = nullptr, *cy_fe = nullptr, *cy_fo = nullptr, *cphys = nullptr, *csums = nullptr;
+    void *ctwz = nullptr, *ctwx = nullptr;
+    FftPlan cpz{}, cpx{};
+    int *cflags = nullptr;
+    std::vector<double> cc_weights; // Clenshaw-Curtis weights / 2 (diagnostics.cpp:17-33)
 
     // workspaces
     void *Ybuf = nullptr, *Zbuf = nullptr, *ZNbuf = nullptr;
@@ -294,6 +327,171 @@ void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out, zd *c
         launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
     }
     forward_zy(c, out, 3, nullptr, add);
+}
+
+// ------------------------------------------------------------- coarse grid ----
+// Transforms::channel_xyzfft (transforms.cpp:452-582): the base grid NX x (NY+1) x NZ of
+// to_physical, statistics and diagnostics, always in FP64, in the product-grid workspaces.
+void ensure_coarse(cdns_ctx *c) {
+    if (c->coarse_ready) return;
+    const Geometry &g = c->g;
+    if (!rf_supported(g.NZ) || !rf_supported(g.NX))
+        throw std::invalid_argument("coarse transform: NX and NZ must be register-FFT lengths");
+    const int NLc = g.NY + 1, NLHc = g.NY / 2 + 1;
+    auto upload = [&](const std::vector<double> &h) {
+        double *d = c->alloc<double>(h.size());
+        CDNS_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+        return d;
+    };
+    auto up_tw = [&](int n) {
+        const auto t = twiddles(n);
+        std::vector<double> dv(t.begin(), t.end());
+        return (void *)upload(dv);
+    };
+    std::vector<double> B, Fe, Fo;
+    y_tables(g.NY, g.nj, NLHc, B, Fe, Fo);
+    c->cy_b = upload(B);
+    c->cy_fe = upload(Fe);
+    c->cy_fo = upload(Fo);
+    c->cpz = make_plan(g.NZ);
+    c->cpx = make_plan(g.NX);
+    c->ctwz = up_tw(g.NZ);
+    c->ctwx = up_tw(g.NX);
+    c->cphys = c->alloc<double>(5 * (size_t)g.NZ * NLc * g.NX);
+    c->csums = c->alloc<double>((size_t)NLc * kPlaneSums);
+    c->cflags = c->alloc<int>(std::max(NLc, 256));
+    // Clenshaw-Curtis weights on [-1, 1] over height 2 (diagnostics.cpp:17-33)
+    const int n = g.NY;
+    const double pi = std::acos(-1.0);
+    c->cc_weights.assign(NLc, 0.0);
+    for (int j = 0; j <= n; ++j) {
+        const double theta = pi * j / n;
+        double weight;
+        if (j == 0 || j == n) weight = 1.0 / (n * n - (n % 2 == 0 ? 1.0 : 0.0));
+        else {
+            double value = 1.0;
+            for (int k = 1; k <= (n - 1) / 2; ++k) value -= 2.0 * std::cos(2 * k * theta) / (4.0 * k * k - 1.0);
+            if (n % 2 == 0) value -= std::cos(n * theta) / (n * n - 1.0);
+            weight = 2.0 * value / n;
+        }
+        c->cc_weights[j] = weight / 2.0;
+    }
+    CDNS_HIP(hipStreamSynchronize(c->st));
+    c->coarse_ready = true;
+}
+YArgs yargs_coarse(cdns_ctx *c) {
+    YArgs a = yargs(c);
+    a.coef = c->cy_b;
+    a.NLH = c->g.NY / 2 + 1;
+    a.NY2 = c->g.NY;
+    a.YS = (size_t)(c->g.NY + 1) * c->g.NQ;
+    return a;
+}
+size_t coarse_zs(const cdns_ctx *c) { return (size_t)(c->g.NY + 1) * c->g.NZ * c->g.nkx; }
+ZArgs zargs_coarse(cdns_ctx *c, const void *in, void *out) {
+    const size_t ys = (size_t)(c->g.NY + 1) * c->g.NQ;
+    ZArgs a = zargs(c, in, out, ys, coarse_zs(c));
+    a.tw = c->ctwz;
+    a.plan = c->cpz;
+    a.NL = c->g.NY + 1;
+    a.rf = 1;
+    a.coarse = 1;
+    return a;
+}
+XArgs xargs_coarse(cdns_ctx *c, const void *in, void *out, double *phys) {
+    XArgs a = xargs(c, in, out, coarse_zs(c), coarse_zs(c), 2 * c->P);
+    a.tw = c->ctwx;
+    a.plan = c->cpx;
+    a.NL = c->g.NY + 1;
+    a.NZ2 = c->g.NZ;
+    a.rf = 1;
+    a.coarse = 1;
+    a.phys = phys;
+    return a;
+}
+// spectral plane -> coarse physical field [iz][iy][ix] (device)
+void coarse_backward(cdns_ctx *c, const zd *plane, double *phys) {
+    ensure_coarse(c);
+    YArgs a = yargs_coarse(c);
+    a.spec[0] = plane;
+    {
+        Launch L(c, "coarse_y_backward");
+        launch_ybwd(a, 1, false, c->st);
+    }
+    {
+        Launch L(c, "coarse_z_backward");
+        launch_zbwd(zargs_coarse(c, c->Ybuf, c->Zbuf), 1, false, c->st);
+    }
+    {
+        Launch L(c, "coarse_x_backward");
+        launch_xbwd_phys(xargs_coarse(c, c->Zbuf, nullptr, phys), false, c->st);
+    }
+}
+// coarse physical field (device) -> spectral plane
+void coarse_forward(cdns_ctx *c, const double *phys, zd *plane) {
+    ensure_coarse(c);
+    {
+        Launch L(c, "coarse_x_forward");
+        launch_xfwd_phys(xargs_coarse(c, nullptr, c->ZNbuf, const_cast<double *>(phys)), false, c->st);
+    }
+    {
+        Launch L(c, "coarse_z_forward");
+        launch_zfwd(zargs_coarse(c, c->ZNbuf, c->Ybuf), 1, false, c->st);
+    }
+    YArgs a = yargs_coarse(c);
+    a.coef = c->cy_fe;
+    a.coef2 = c->cy_fo;
+    a.spec_out[0] = plane;
+    a.add[0] = nullptr;
+    {
+        Launch L(c, "coarse_y_forward");
+        launch_yfwd(a, 1, false, c->st);
+    }
+}
+size_t coarse_points(const cdns_ctx *c) { return (size_t)c->g.NZ * (c->g.NY + 1) * c->g.NX; }
+// FieldIO::to_physical (field_io.cpp:199-220) of the state into cphys[0..3] (u, v, w, pressure);
+// returns whether every retained spectral coefficient is finite
+bool state_to_physical(cdns_ctx *c) {
+    ensure_coarse(c);
+    const zd *src[4] = {c->u[0], c->u[1], c->u[2], c->P_};
+    for (int k = 0; k < 4; ++k) coarse_backward(c, src[k], c->cphys + k * coarse_points(c));
+    int ok = 1;
+    const int nb = 256;
+    std::vector<int> flags(nb);
+    for (int k = 0; k < 4; ++k) {
+        launch_planes_finite(src[k], 1, c->g.NQ, c->g.JS, c->g.NYS, c->cflags, nb, c->st);
+        CDNS_HIP(hipMemcpyAsync(flags.data(), c->cflags, nb * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        for (int f : flags) ok &= f;
+    }
+    return ok != 0;
+}
+// Statistics::reduce over cphys (and the plane sum of div^2 when with_div) into host sums
+void plane_moments(cdns_ctx *c, bool higher, bool with_div, std::vector<double> &sums, std::vector<int> &finite) {
+    const Geometry &g = c->g;
+    const int NLc = g.NY + 1;
+    const size_t np = coarse_points(c);
+    DiagArgs a{};
+    a.pu = c->cphys;
+    a.pv = c->cphys + np;
+    a.pw = c->cphys + 2 * np;
+    a.pp = c->cphys + 3 * np;
+    a.pdiv = with_div ? c->cphys + 4 * np : nullptr;
+    a.NX = g.NX;
+    a.NZ = g.NZ;
+    a.NL = NLc;
+    a.higher = higher ? 1 : 0;
+    a.sums = c->csums;
+    a.finite = c->cflags;
+    {
+        Launch L(c, "plane_moments");
+        launch_plane_moments(a, c->st);
+    }
+    sums.resize((size_t)NLc * kPlaneSums);
+    finite.resize(NLc);
+    CDNS_HIP(hipMemcpyAsync(sums.data(), c->csums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    CDNS_HIP(hipMemcpyAsync(finite.data(), c->cflags, NLc * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    CDNS_HIP(hipStreamSynchronize(c->st));
 }
 
 void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
@@ -789,25 +987,8 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         C->WR = upload(WR);
 
         // y-stage matrices from exact-angle cosines
-        const int N = g.NY2;
-        const auto cs = cospi_table(N);
-        std::vector<double> B((size_t)g.nj * g.NLH);
-        for (int j = 0; j < g.nj; ++j)
-            for (int l = 0; l < g.NLH; ++l) {
-                const long double bw = (j == N) ? 0.5L : 1.0L; // REDFT00 endpoint weight x the 0.5 z factor
-                B[(size_t)j * g.NLH + l] = (double)(bw * cs[((long long)j * l) % (2 * N)]);
-            }
-        const int nje = (g.nj + 1) / 2, njo = g.nj / 2;
-        std::vector<double> Fe((size_t)g.NLH * nje), Fo((size_t)g.NLH * std::max(njo, 1));
-        for (int l = 0; l < g.NLH; ++l) {
-            const long double wl = (l == 0) ? 1.0L : 2.0L;
-            for (int j = 0; j < g.nj; ++j) {
-                const long double s = j == 0 ? 1.0L / (2.0L * N) : 1.0L / N;
-                const double v = (double)(s * wl * cs[((long long)j * l) % (2 * N)]);
-                if (j % 2 == 0) Fe[(size_t)l * nje + j / 2] = v;
-                else Fo[(size_t)l * njo + j / 2] = v;
-            }
-        }
+        std::vector<double> B, Fe, Fo;
+        y_tables(g.NY2, g.nj, g.NLH, B, Fe, Fo);
         C->ycoef_b = upload(B);
         C->ycoef_fe = upload(Fe);
         C->ycoef_fo = upload(Fo);
@@ -983,6 +1164,117 @@ int cdns_xyzfft2(cdns_ctx *c, char dir, double *phys, double *spec, int stride, 
             planes_to_ref(c, cp, 1, g.nj, spec, stride, where, g.JS);
         }
         if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_xyzfft(cdns_ctx *c, char dir, double *phys, double *spec, int stride, int where) {
+    try {
+        if (stride < 1) throw std::invalid_argument("spectral stride must be positive");
+        if (dir != 'F' && dir != 'B') throw std::invalid_argument("direction must be 'F' or 'B'");
+        ensure_coarse(c);
+        const Geometry &g = c->g;
+        const size_t np = coarse_points(c);
+        double *dphys = where == CDNS_DEVICE ? phys : c->cphys + 4 * np;
+        if (dir == 'B') {
+            zd *pl[1] = {c->scr[0]};
+            ref_to_planes(c, spec, stride, 1, g.nj, pl, where, g.JS);
+            coarse_backward(c, c->scr[0], dphys);
+            if (where == CDNS_HOST)
+                CDNS_HIP(hipMemcpyAsync(phys, dphys, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        } else {
+            if (where == CDNS_HOST)
+                CDNS_HIP(hipMemcpyAsync(dphys, phys, np * sizeof(double), hipMemcpyHostToDevice, c->st));
+            coarse_forward(c, dphys, c->scr[0]);
+            const zd *cp[1] = {c->scr[0]};
+            planes_to_ref(c, cp, 1, g.nj, spec, stride, where, g.JS);
+        }
+        if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_to_physical(cdns_ctx *c, double *velocity, double *pressure, int *finite, int where) {
+    try {
+        const bool ok = state_to_physical(c);
+        const size_t np = coarse_points(c);
+        const hipMemcpyKind kind = where == CDNS_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        if (velocity) CDNS_HIP(hipMemcpyAsync(velocity, c->cphys, 3 * np * sizeof(double), kind, c->st));
+        if (pressure) CDNS_HIP(hipMemcpyAsync(pressure, c->cphys + 3 * np, np * sizeof(double), kind, c->st));
+        CDNS_HIP(hipStreamSynchronize(c->st));
+        if (finite) *finite = ok ? 1 : 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_plane_moments(cdns_ctx *c, int higher, double *sums, int *finite) {
+    try {
+        state_to_physical(c);
+        std::vector<double> s;
+        std::vector<int> f;
+        plane_moments(c, higher != 0, false, s, f);
+        const int NLc = c->g.NY + 1;
+        for (int iy = 0; iy < NLc; ++iy) {
+            for (int m = 0; m < 20; ++m) sums[(size_t)iy * 20 + m] = s[(size_t)iy * kPlaneSums + m];
+            if (finite) finite[iy] = f[iy];
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int cdns_diagnostics(cdns_ctx *c, double out[3], int *finite) {
+    try {
+        const Geometry &g = c->g;
+        bool ok = state_to_physical(c);
+        DiagArgs d{};
+        d.u = c->u[0];
+        d.v = c->u[1];
+        d.w = c->u[2];
+        d.out = c->scr[0];
+        d.WAVX = c->WAVX;
+        d.WAVZ = c->WAVZ;
+        d.NQ = g.NQ;
+        d.nzk = g.nzk;
+        d.JS = g.JS;
+        d.ny = g.NYS;
+        {
+            Launch L(c, "divergence");
+            launch_divergence(d, g.E, c->st);
+        }
+        coarse_backward(c, c->scr[0], c->cphys + 4 * coarse_points(c));
+        std::vector<double> s;
+        std::vector<int> f;
+        plane_moments(c, false, true, s, f);
+        // Diagnostics::evaluate (diagnostics.cpp:39-75)
+        const double inv_plane = 1.0 / ((double)g.NX * g.NZ);
+        double ekt = 0.0, tke = 0.0, div2 = 0.0;
+        for (int iy = 0; iy <= g.NY; ++iy) {
+            const double *p = &s[(size_t)iy * kPlaneSums];
+            ok = ok && f[iy];
+            ekt += p[5];
+            double variance = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double mean = p[k] * inv_plane;
+                const double raw = p[4 + k] * inv_plane - mean * mean;
+                ok = ok && std::isfinite(raw);
+                variance += std::max(0.0, raw);
+            }
+            tke += 0.5 * c->cc_weights[iy] * variance;
+            div2 += c->cc_weights[iy] * inv_plane * p[20];
+        }
+        out[0] = ekt;
+        out[1] = tke;
+        out[2] = std::sqrt(div2);
+        ok = ok && std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2]);
+        if (finite) *finite = ok ? 1 : 0;
         return 0;
     } catch (const std::exception &e) {
         return fail(c, e);

@@ -30,6 +30,7 @@ struct ZArgs {
     int rf; // use the register-resident FFT when available
     int nkxg; // register z forward: kx groups per (field, l)
     int pair; // FFT rows on lane pairs (even factors)
+    int coarse; // coarse-grid transform: at NZS = N/2 the kz = +-NZS pair shares slot NZS
 };
 
 // Layout of the z-physical fields produced by the x forward stages and read by the z forward:
@@ -50,6 +51,7 @@ struct XArgs {
     int nkx, NL, NZ2, P;
     int rf;
     int pair; // product: rows of the register FFT on lane pairs (even factors)
+    int coarse; // coarse-grid transform: at NXS = N/2 the Nyquist coefficient counts twice
     double *phys; // reference-layout physical field for the single-field stages
     // product a x b: input fields a0 a1 a2 b0 b1 b2 and the three output fields
     const void *fin[6];
@@ -164,6 +166,23 @@ void launch_adj_solve(const AdjArgs &a, int E, hipStream_t st);
 void launch_adj_combine(const AdjArgs &a, int E, hipStream_t st);
 void launch_adj_fold(const AdjArgs &a, hipStream_t st);
 void launch_objective(const AdjArgs &a, int E, hipStream_t st);
+
+// diagnostics on the coarse physical grid
+constexpr int kPlaneSums = 21; // the 20 sums of Statistics::reduce and the plane sum of div^2
+struct DiagArgs {
+    const zd *u, *v, *w; // spectral planes
+    zd *out;
+    const double *WAVX, *WAVZ;
+    int NQ, nzk, JS, ny;
+    const double *pu, *pv, *pw, *pp, *pdiv; // physical [iz][iy][ix]; pdiv may be null
+    int NX, NZ, NL, higher;
+    double *sums; // [NL][kPlaneSums]
+    int *finite;  // [NL]
+};
+void launch_divergence(const DiagArgs &a, int E, hipStream_t st);
+void launch_planes_finite(const zd *planes, size_t nplanes, int NQ, int JS, int ny, int *flags, int nblocks,
+                          hipStream_t st);
+void launch_plane_moments(const DiagArgs &a, hipStream_t st);
 
 void launch_solve(const SolveArgs &a, int E, hipStream_t st);
 // real tables of the compact form from the complex setup tables

@@ -271,7 +271,12 @@ __global__ void __launch_bounds__(64) k_xbwd_phys_rf(XArgs a) {
     if (t < N1 && pa < np) {
         const cpx<R> *A = Z + (size_t)pa * a.nkx;
         const cpx<R> *Bv = pb < np ? Z + (size_t)pb * a.nkx : nullptr;
-        sfor<0, N2>([&](auto n2) { x[n2] = herm_pack(A, Bv, t + N1 * decltype(n2)::value, N, K); });
+        sfor<0, N2>([&](auto n2) {
+            const int k = t + N1 * decltype(n2)::value;
+            x[n2] = herm_pack(A, Bv, k, N, K);
+            // coarse grid, NXS = N/2: the Nyquist coefficient doubled (transforms.cpp:576-578)
+            if (a.coarse && 2 * k == N) x[n2] = {R(2) * x[n2].x, R(2) * x[n2].y};
+        });
     }
     two_pass<R, N1, N2, +1>(x, t, tw, buf[g], wsync);
     if (t < N2) {
@@ -326,8 +331,10 @@ __global__ void __launch_bounds__(64) k_xfwd_phys_rf(XArgs a) {
     for (int k = t; k <= K; k += G) {
         const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
         const cpx<R> s = f + gm, d = f - gm;
-        if (pa < np) ZN[zline_index(pa, k, a.nkx, a.NZ2)] = {half * s.x, half * s.y};
-        if (pb < np) ZN[zline_index(pb, k, a.nkx, a.NZ2)] = {half * d.y, -half * d.x};
+        // coarse grid, NXS = N/2: the Nyquist coefficient halved (transforms.cpp:496-497)
+        const R hk = (a.coarse && 2 * k == N) ? R(0.5) * half : half;
+        if (pa < np) ZN[zline_index(pa, k, a.nkx, a.NZ2)] = {hk * s.x, hk * s.y};
+        if (pb < np) ZN[zline_index(pb, k, a.nkx, a.NZ2)] = {hk * d.y, -hk * d.x};
     }
 }
 
@@ -380,6 +387,7 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
             else if (s >= N - NZS) kz = s - N;
             if (s == N - NZS && s <= NZS) kz = -NZS;
             if (kz != (1 << 30)) x[n2] = Y[kz + NZS];
+            if (a.coarse && 2 * s == N && s == NZS) x[n2] = Y[2 * NZS] + Y[0];
         });
     }
     two_pass<R, N1, N2, +1>(x, t, tw, lds + line * LB, wsync);
@@ -421,9 +429,11 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
         cpx<R> *Y = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * a.NQ + (size_t)kx * a.nzk;
         sfor<0, N1>([&](auto k1) {
             const int s = t + N2 * decltype(k1)::value;
-            // when N = 2 NZS the slot NZS serves both kz = NZS and kz = -NZS
-            if (s <= NZS) Y[s + NZS] = inv * x[k1];
-            if (s >= N - NZS) Y[s - N + NZS] = inv * x[k1];
+            // when N = 2 NZS the slot NZS serves both kz = NZS and kz = -NZS, halved on the
+            // coarse grid (transforms.cpp:507-510)
+            const R sc = (a.coarse && 2 * s == N) ? R(0.5) * inv : inv;
+            if (s <= NZS) Y[s + NZS] = sc * x[k1];
+            if (s >= N - NZS) Y[s - N + NZS] = sc * x[k1];
         });
     }
 }
@@ -473,6 +483,8 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
             else if (s >= N - NZS) kz = s - N;
             if (s == N - NZS && s <= NZS) kz = -NZS;
             if (kz != (1 << 30)) x[m] = Y[kz + NZS];
+            // coarse grid, NZS = N/2: the slot holds X(NZS) + X(-NZS) (transforms.cpp:561-564)
+            if (a.coarse && 2 * s == N && s == NZS) x[m] = Y[2 * NZS] + Y[0];
         });
     }
     load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap

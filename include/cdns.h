@@ -58,6 +58,24 @@ void *cdns_stream(cdns_ctx *ctx);
  * spectral component at spec[0], spec[stride], ...; 'F' writes only that component. */
 int cdns_xyzfft2(cdns_ctx *ctx, char dir, double *phys, double *spec, int stride, int where);
 
+/* Transforms::channel_xyzfft (src/fft/transforms.hpp:39, transforms.cpp:452-582): the coarse
+ * grid NX x (NY+1) x NZ, physical layout [iz][iy][ix], with the Nyquist rules of NXS = NX/2 and
+ * NZS = NZ/2; same stride and "write only the selected component" contract as cdns_xyzfft2. */
+int cdns_xyzfft(cdns_ctx *ctx, char dir, double *phys, double *spec, int stride, int where);
+
+/* FieldIO::to_physical (src/io/field_io.cpp:199-220): the state's velocity (three consecutive
+ * [iz][iy][ix] arrays) and pressure on the coarse grid; finite = every retained spectral
+ * coefficient of both is finite.  Either output may be NULL. */
+int cdns_to_physical(cdns_ctx *ctx, double *velocity, double *pressure, int *finite, int where);
+/* Statistics::reduce (src/sim/statistics.cpp:54-109) on the state: per plane iy the 20 sums
+ * u v w p, uu vv ww, uv vw wu, pu pv pw pp, u^3 v^3 w^3, u^4 v^4 w^4 (p = pressure - |u|^2/2;
+ * the last 13 only when higher != 0) into sums[(NY+1) x 20], host memory; finite[NY+1] or NULL. */
+int cdns_plane_moments(cdns_ctx *ctx, int higher, double *sums, int *finite);
+/* Diagnostics::evaluate (src/sim/diagnostics.cpp:39-75) on the state: out = {EKT = sum v^2 over
+ * the grid nodes, TKE = Clenshaw-Curtis volume mean of 1/2 sum_c var_xz(u_c), L2 norm of the
+ * physical divergence}, finite as the reference combines it. */
+int cdns_diagnostics(cdns_ctx *ctx, double out[3], int *finite);
+
 /* vector_ops::curl and vector_ops::cross_product (src/solver/vector_ops.hpp:22,33). */
 int cdns_curl(cdns_ctx *ctx, double *out, const double *in, int where);
 int cdns_cross_product(cdns_ctx *ctx, double *out, const double *a, const double *b, int where);

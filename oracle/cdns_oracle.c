@@ -1181,3 +1181,105 @@ uint64_t orc_mt64_next(orc_mt64 *g) {
     y ^= (y >> 43);
     return y;
 }
+
+/* ---------------------------------------------------------- statistics / diagnostics ---- */
+
+/* Statistics::reduce plane sums (statistics.cpp:54-90): four interleaved lanes per x row, lanes
+ * combined as (l0 + l1) + (l2 + l3); p = raw pressure - |u|^2/2.  Physical arrays [iz][iy][ix]. */
+void orc_plane_moments(const orc_ctx *c, const double *u, const double *v, const double *w, const double *pr,
+                       int higher, double *sums, int *finite) {
+    const int NX = c->cfg.NX, NZ = c->cfg.NZ, NL = c->cfg.NY + 1;
+#pragma omp parallel for schedule(static)
+    for (int iy = 0; iy < NL; ++iy) {
+        double acc[20][4];
+        memset(acc, 0, sizeof acc);
+        int ok = 1;
+        for (int iz = 0; iz < NZ; ++iz) {
+            const size_t base = ((size_t)iz * NL + iy) * NX;
+            for (int ix = 0; ix < NX; ix += 4)
+                for (int lane = 0; lane < 4 && ix + lane < NX; ++lane) {
+                    const size_t i = base + ix + lane;
+                    const double U = u[i], V = v[i], W = w[i];
+                    const double uu = U * U, vv = V * V, ww = W * W;
+                    const double raw = pr[i];
+                    const double P = raw - 0.5 * (uu + vv + ww);
+                    ok &= isfinite(U) && isfinite(V) && isfinite(W) && isfinite(raw) && isfinite(P);
+                    acc[0][lane] += U; acc[1][lane] += V; acc[2][lane] += W; acc[3][lane] += P;
+                    acc[4][lane] += uu; acc[5][lane] += vv; acc[6][lane] += ww;
+                    if (higher) {
+                        acc[7][lane] += U * V; acc[8][lane] += V * W; acc[9][lane] += W * U;
+                        acc[10][lane] += P * U; acc[11][lane] += P * V; acc[12][lane] += P * W;
+                        acc[13][lane] += P * P;
+                        acc[14][lane] += uu * U; acc[15][lane] += vv * V; acc[16][lane] += ww * W;
+                        acc[17][lane] += uu * uu; acc[18][lane] += vv * vv; acc[19][lane] += ww * ww;
+                    }
+                }
+        }
+        for (int m = 0; m < 20; ++m) {
+            sums[(size_t)iy * 20 + m] = (acc[m][0] + acc[m][1]) + (acc[m][2] + acc[m][3]);
+            ok &= isfinite(sums[(size_t)iy * 20 + m]);
+        }
+        finite[iy] = ok;
+    }
+}
+
+/* Diagnostics weights (diagnostics.cpp:17-33): Clenshaw-Curtis on [-1, 1], divided by 2. */
+void orc_cc_weights(const orc_ctx *c, double *wt) {
+    const int n = c->cfg.NY;
+    const double pi = acos(-1.0);
+    for (int j = 0; j <= n; ++j) {
+        const double theta = pi * j / n;
+        double weight;
+        if (j == 0 || j == n) weight = 1.0 / (n * n - (n % 2 == 0 ? 1.0 : 0.0));
+        else {
+            double value = 1.0;
+            for (int k = 1; k <= (n - 1) / 2; ++k) value -= 2.0 * cos(2 * k * theta) / (4.0 * k * k - 1.0);
+            if (n % 2 == 0) value -= cos(n * theta) / (n * n - 1.0);
+            weight = 2.0 * value / n;
+        }
+        wt[j] = weight / 2.0;
+    }
+}
+
+/* Diagnostics::evaluate (diagnostics.cpp:39-75): out = {EKT, TKE, divergence L2} from the plane
+ * sums of the physical fields and the spectral velocity (reference layout, interleaved). */
+int orc_diagnostics(orc_ctx *c, const double *sums, const int *plane_finite, const double *velocity,
+                    int spectral_finite, double out[3]) {
+    const int NX = c->cfg.NX, NZ = c->cfg.NZ, NL = c->cfg.NY + 1;
+    double *div = (double *)calloc(c->S * 2, sizeof(double));
+    double *phys = (double *)malloc((size_t)NX * NZ * NL * sizeof(double));
+    double *wt = (double *)malloc(NL * sizeof(double));
+    orc_divergence(c, div, velocity);
+    coarse_backward(c, phys, (const cplx *)div, 1);
+    orc_cc_weights(c, wt);
+    int finite = spectral_finite;
+    const double inv_plane = 1.0 / ((double)NX * NZ);
+    double ekt = 0.0, tke = 0.0, div_square = 0.0;
+    for (int iy = 0; iy < NL; ++iy) {
+        const double *p = sums + (size_t)iy * 20;
+        finite &= plane_finite[iy];
+        ekt += p[5];
+        double variance = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double mean = p[k] * inv_plane;
+            const double raw = p[4 + k] * inv_plane - mean * mean;
+            finite &= isfinite(raw);
+            variance += raw > 0.0 ? raw : 0.0;
+        }
+        tke += 0.5 * wt[iy] * variance;
+        for (int iz = 0; iz < NZ; ++iz)
+            for (int ix = 0; ix < NX; ++ix) {
+                const double d = phys[((size_t)iz * NL + iy) * NX + ix];
+                finite &= isfinite(d);
+                div_square += wt[iy] * inv_plane * d * d;
+            }
+    }
+    out[0] = ekt;
+    out[1] = tke;
+    out[2] = sqrt(div_square);
+    finite &= isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2]);
+    free(div);
+    free(phys);
+    free(wt);
+    return finite;
+}

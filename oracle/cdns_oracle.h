@@ -46,6 +46,14 @@ int orc_xyzfft2(orc_ctx *c, char dir, double *phys, double *spec, int stride);
 int orc_xyzfft(orc_ctx *c, char dir, double *phys, double *spec, int stride);
 
 /* vector_ops (vector_ops.cpp). */
+/* Statistics::reduce plane sums (statistics.cpp:54-90) of physical [iz][iy][ix] fields into
+ * sums[(NY+1) x 20]; Diagnostics weights (diagnostics.cpp:17-33) and evaluate (:39-75); returns
+ * the finite flag. */
+void orc_plane_moments(const orc_ctx *c, const double *u, const double *v, const double *w, const double *pr,
+                       int higher, double *sums, int *finite);
+void orc_cc_weights(const orc_ctx *c, double *wt);
+int orc_diagnostics(orc_ctx *c, const double *sums, const int *plane_finite, const double *velocity,
+                    int spectral_finite, double out[3]);
 void orc_derivative(orc_ctx *c, int axis, double *out, const double *in);
 void orc_curl(orc_ctx *c, double *out, const double *in);
 void orc_divergence(orc_ctx *c, double *out, const double *in);

@@ -79,6 +79,10 @@ def lib():
         L.orc_adj_objective.restype = d
         L.orc_adj_run.argtypes = [vp, vp, vp, i, i, i, vp, vp]
         L.orc_adj_run.restype = d
+        L.orc_plane_moments.argtypes = [vp, vp, vp, vp, vp, i, vp, vp]
+        L.orc_cc_weights.argtypes = [vp, vp]
+        L.orc_diagnostics.argtypes = [vp, vp, vp, vp, i, vp]
+        L.orc_diagnostics.restype = i
         L.orc_mt64_seed.argtypes = [ctypes.POINTER(Mt64), ctypes.c_uint64]
         L.orc_mt64_next.argtypes = [ctypes.POINTER(Mt64)]
         L.orc_mt64_next.restype = ctypes.c_uint64
@@ -163,6 +167,38 @@ class Oracle:
         rc = lib().orc_xyzfft(self._h, direction.encode(), _p(phys), ptr, int(stride))
         if rc != 0:
             raise ValueError(lib().orc_last_error().decode())
+
+    def to_physical(self, velocity, pressure):
+        """FieldIO::to_physical on the coarse grid: (velocity [3][NZ][NY+1][NX], pressure, finite)."""
+        m = self.cfg
+        vel = np.zeros((3, m["NZ"], m["NY"] + 1, m["NX"]), np.float64)
+        pr = self.coarse_phys()
+        v = np.ascontiguousarray(velocity)
+        for k in range(3):
+            self.xyzfft("B", vel[k], v, stride=3, offset=k)
+        self.xyzfft("B", pr, np.ascontiguousarray(pressure))
+        finite = bool(np.isfinite(v.view(np.float64)).all() and np.isfinite(np.asarray(pressure).view(np.float64)).all())
+        return vel, pr, finite
+
+    def plane_moments(self, u, v, w, p, higher=True):
+        n = self.cfg["NY"] + 1
+        sums = np.zeros((n, 20), np.float64)
+        fin = np.zeros(n, np.int32)
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        lib().orc_plane_moments(self._h, *[_p(a) for a in args], int(bool(higher)), _p(sums), _p(fin))
+        return sums, fin.astype(bool)
+
+    def cc_weights(self):
+        wt = np.zeros(self.cfg["NY"] + 1)
+        lib().orc_cc_weights(self._h, _p(wt))
+        return wt
+
+    def diagnostics(self, sums, plane_finite, velocity, spectral_finite=True):
+        out = np.zeros(3)
+        pf = np.ascontiguousarray(plane_finite, dtype=np.int32)
+        fin = lib().orc_diagnostics(self._h, _p(np.ascontiguousarray(sums)), _p(pf), _p(np.ascontiguousarray(velocity)),
+                                    int(bool(spectral_finite)), _p(out))
+        return {"ekt": out[0], "tke": out[1], "divergence_l2": out[2], "finite": bool(fin)}
 
     def derivative(self, axis, f):
         out = self.scalar()
