@@ -26,10 +26,19 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("name", ["channel_32", "stage3"])
+def _config(name):
+    if name == "channel_32":
+        return config.channel_32()
+    if name == "long_columns":
+        # NYS = 150: four entries of each parity chain per lane pair in the adjoint solve
+        return config.make(16, 160, 16, 8, 150, 8, 24, 240, 24, 2 * math.pi, math.pi, 180.0, 5e-4)
+    return config.reference_example()
+
+
+@pytest.mark.parametrize("name", ["channel_32", "stage3", "long_columns"])
 def test_adjoint_step_matches_oracle(gpu, name):
     """One transposed step (iterations 5, 1, 2) on the same state and adjoint inputs."""
-    cfg = config.channel_32() if name == "channel_32" else config.reference_example()
+    cfg = _config(name)
     rng = np.random.default_rng(21)
     o = pyoracle.Oracle(cfg)
     A = pyoracle.Adjoint(o)

@@ -3,6 +3,8 @@
 Tolerances: transforms and single operators 1e-13 (max-abs relative), full steps
 1e-11 relative L2 after one step and 1e-9 after 20 steps on every velocity component.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -198,3 +200,52 @@ def test_step_parity_retau180(gpu, steps, tol):
     errs = fields.rel_l2_components(got, want)
     print(f"retau180 {steps} steps rel-L2 per component: {errs}")
     assert max(errs) < tol, errs
+
+
+def _tall_config():
+    """Long columns (NYS = 150 > 127: four entries of each parity chain per lane) on a small x-z
+    grid, with the product grid on the LDS Stockham FFTs."""
+    return config.make(16, 160, 16, 8, 150, 8, 24, 240, 24, 2 * math.pi, math.pi, 180.0, 5e-4)
+
+
+@pytest.mark.parametrize("steps,tol", [(1, 1e-11), (5, 1e-10)])
+def test_step_parity_long_columns(gpu, steps, tol):
+    cfg = _tall_config()
+    u0 = fields.seeded_velocity(cfg, seed=150, mean="log", spectrum="k53", amplitude=0.1, kcut=4)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state("velocity", u0)
+    for it in range(1, steps + 1):
+        ref.step(it)
+    dev = gpu.Channel(cfg)
+    dev.set_state("velocity", u0)
+    dev.run(steps, it_begin=1)
+    errs = fields.rel_l2_components(dev.get_state("velocity"), ref.get_state("velocity"))
+    print(f"long columns {steps} steps rel-L2 per component: {errs}")
+    assert max(errs) < tol, errs
+
+
+def test_step_parity_retau395(gpu):
+    """The larger workload (256x193x192, product 384x289x288; x and z on the LDS Stockham FFTs):
+    one FP64 step vs the oracle, and the mixed form finite and close after five steps."""
+    cfg = config.retau395()
+    u0 = fields.seeded_velocity(cfg, seed=395, mean="log", spectrum="k53", amplitude=0.1)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state("velocity", u0)
+    ref.step(1)
+    dev = gpu.Channel(cfg)
+    dev.set_state("velocity", u0)
+    dev.run(1, it_begin=1)
+    errs = fields.rel_l2_components(dev.get_state("velocity"), ref.get_state("velocity"))
+    print(f"retau395 1 step rel-L2 per component: {errs}")
+    assert max(errs) < 1e-11, errs
+    dev.run(4, it_begin=2)
+    f64 = dev.get_state("velocity")
+    dev.close()
+    mx = gpu.Channel(cfg, precision=gpu.MIXED)
+    mx.set_state("velocity", u0)
+    mx.run(5, it_begin=1)
+    mixed = mx.get_state("velocity")
+    assert np.isfinite(mixed).all()
+    errs = fields.rel_l2_components(mixed, f64)
+    print(f"retau395 mixed vs FP64 after 5 steps: {errs}")
+    assert max(errs) < 1e-5, errs
