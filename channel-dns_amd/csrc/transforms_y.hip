@@ -17,6 +17,7 @@
 // i = 4*r + kk for FP64 (Mfma<R>::row).  The forward kernel orders its data columns so that
 // each lane ends with (re, im) pairs of the same column in registers (0,1) and (2,3).
 #include "internal.hpp"
+#include "regfft.hpp"
 
 #include <algorithm>
 
@@ -75,15 +76,23 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
     __syncthreads();
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 15, k = lane >> 4;
+    const int col = lane & 15, k = lane >> 4, hh = col & 1;
     const int qb = (cg * kWaves + wv) * 8 * CT;
-    const double *src[CT];
+    // FP32: lane (col, k) loads the whole complex S[q][8s + 2k + (col & 1)], q = qb + 8c + col/2,
+    // so that each load instruction reads eight columns x 128 contiguous bytes; the MFMA operands
+    // are one part of the even-j and one of the odd-j value, its own and its pair partner's (DPP
+    // lane swap).  FP64: lane (col, k) loads part col & 1 of S[q][8s + 2k] and S[q][8s + 2k + 1]
+    // (two 8-byte loads), which measures faster there than the 16-byte loads and lane swaps.
+    constexpr bool kFused = sizeof(R) == 4;
+    const zd *src[CT];
+    const double *srcp[CT];
     bool okq[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
         const int q = qb + 8 * c + (col >> 1);
         okq[c] = q < a.NQ;
-        src[c] = reinterpret_cast<const double *>(a.spec[f]) + ((size_t)(okq[c] ? q : 0) * a.JS) * 2 + (col & 1);
+        src[c] = a.spec[f] + (size_t)(okq[c] ? q : 0) * a.JS;
+        srcp[c] = reinterpret_cast<const double *>(a.spec[f]) + ((size_t)(okq[c] ? q : 0) * a.JS) * 2 + (col & 1);
     }
     v4 accE[CT][TRP], accO[CT][TRP];
     R mid[CT];
@@ -98,13 +107,21 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
     }
     // k-step s covers j = 8s .. 8s+7: this lane's even j = 8s + 2k, odd j = 8s + 2k + 1
     const int nsteps = a.njp / 8;
+    zd bv[CT];
     R bE[CT], bO[CT];
     auto load = [&](int s) {
-        const int je = 8 * s + 2 * k, jo = je + 1;
+        const int je = 8 * s + 2 * k;
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            bE[c] = (okq[c] && je < nj) ? R(src[c][2 * je]) : R(0);
-            bO[c] = (okq[c] && jo < nj) ? R(src[c][2 * jo]) : R(0);
+            if constexpr (kFused) {
+                const int j = je + hh;
+                const zd t = src[c][j < nj ? j : 0];
+                bv[c] = (okq[c] && j < nj) ? t : zd{0.0, 0.0};
+            } else {
+                const int jo = je + 1;
+                bE[c] = (okq[c] && je < nj) ? R(srcp[c][2 * je]) : R(0);
+                bO[c] = (okq[c] && jo < nj) ? R(srcp[c][2 * jo]) : R(0);
+            }
         }
     };
     load(0);
@@ -112,8 +129,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
         R cE[CT], cO[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            cE[c] = bE[c];
-            cO[c] = bO[c];
+            if constexpr (kFused) {
+                const R vx = R(bv[c].x), vy = R(bv[c].y);
+                const R px = pair_swap(vx), py = pair_swap(vy);
+                // even lanes: Re S[j even] (own), Re S[j odd] (partner); odd lanes: Im S[j even]
+                // (partner), Im S[j odd] (own)
+                cE[c] = hh ? py : vx;
+                cO[c] = hh ? vy : px;
+            } else {
+                cE[c] = bE[c];
+                cO[c] = bO[c];
+            }
         }
         if (s + 1 < nsteps) load(s + 1);
         const int je = 8 * s + 2 * k;
