@@ -10,6 +10,8 @@ The classes mirror the reference's interfaces for this path:
   TimeStepper.nonlinear_term      src/solver/timestepper.hpp:10-11
   EqSolvePV.setup / solve_modes   src/solver/eqsolve_pv.hpp:17-51
   Simulation state, step, run     src/sim/simulation.hpp:21-62
+  initial_bc (seeded fields)      src/sim/initial_bc.cpp:15-99        (module initial_bc)
+  FieldIO restart / physical / EKT src/io/field_io.cpp:121-156,222-427 (module field_io)
 Arrays are numpy (host) or torch CUDA tensors (device) in the reference layouts.
 The library is required: nothing here computes the step on the CPU.
 """
@@ -33,7 +35,7 @@ ABI_FUNCTIONS = [
     "cdns_set_state", "cdns_get_state", "cdns_step", "cdns_run", "cdns_flow_gradients", "cdns_time",
     "cdns_profile", "cdns_profile_read", "cdns_profile_reset", "cdns_set_body_force", "cdns_objective",
     "cdns_adjoint_step", "cdns_adjoint_run", "cdns_xyzfft", "cdns_to_physical", "cdns_plane_moments",
-    "cdns_diagnostics",
+    "cdns_diagnostics", "cdns_engine_create", "cdns_engine_draw", "cdns_engine_destroy",
 ]
 
 
@@ -87,6 +89,11 @@ def lib(path=None):
         L.cdns_to_physical.argtypes = [vp, vp, vp, vp, i]
         L.cdns_plane_moments.argtypes = [vp, i, vp, vp]
         L.cdns_diagnostics.argtypes = [vp, vp, vp]
+        L.cdns_engine_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+        L.cdns_engine_create.restype = i
+        L.cdns_engine_draw.argtypes = [vp, i, vp, ctypes.c_ulonglong]
+        L.cdns_engine_draw.restype = i
+        L.cdns_engine_destroy.argtypes = [vp]
         for n in ("cdns_xyzfft2", "cdns_curl", "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup",
                   "cdns_solve_modes", "cdns_get_table", "cdns_set_state", "cdns_get_state", "cdns_step",
                   "cdns_run", "cdns_flow_gradients", "cdns_profile", "cdns_profile_read", "cdns_profile_reset",

@@ -84,4 +84,8 @@ def load_toml(path):
                b_pre0=parm["b_pre0"], a_V=parm["a_v"], b_V=parm["b_v"], V_Microblow0=ctrl["v_microblow0"])
     cfg["NSTEPS"] = int(ctrl["nsteps"])
     cfg["INDEX_Serial"] = int(mesh["index_serial"])
+    # initial field (initial_bc.cpp:66-95): fresh start or restart, and the disturbance
+    cfg["INDEX_CONT"] = int(ctrl["index_cont"])
+    cfg["EPS_init"] = float(ctrl["eps_init"])
+    cfg["INDEX_disturb_Z"] = int(mesh["index_disturb_z"])
     return cfg

@@ -9,6 +9,8 @@
 #include <exception>
 #include <map>
 #include <memory>
+#include <new>
+#include <random>
 
 using namespace cdns;
 
@@ -1114,6 +1116,32 @@ void cdns_destroy(cdns_ctx *c) {
 }
 
 const char *cdns_last_error(const cdns_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+struct cdns_engine {
+    std::mt19937_64 g;
+};
+
+int cdns_engine_create(unsigned long long seed, cdns_engine **out) {
+    if (!out) return 1;
+    *out = new (std::nothrow) cdns_engine{std::mt19937_64(seed)};
+    return *out ? 0 : 1;
+}
+
+int cdns_engine_draw(cdns_engine *e, int kind, void *out, unsigned long long n) {
+    if (!e || (!out && n)) return 1;
+    if (kind == 0) {
+        auto *o = static_cast<unsigned long long *>(out);
+        for (unsigned long long i = 0; i < n; ++i) o[i] = e->g();
+    } else if (kind == 1) {
+        auto *o = static_cast<double *>(out);
+        for (unsigned long long i = 0; i < n; ++i) o[i] = static_cast<double>(e->g() >> 11) * 0x1.0p-53 - 0.5;
+    } else {
+        return 1;
+    }
+    return 0;
+}
+
+void cdns_engine_destroy(cdns_engine *e) { delete e; }
 
 void *cdns_stream(cdns_ctx *c) { return c ? (void *)c->st : nullptr; }
 

@@ -145,6 +145,17 @@ int cdns_adjoint_step(cdns_ctx *ctx, int it, const double *velocity, double *adj
 int cdns_adjoint_run(cdns_ctx *ctx, const double *u0, const double *force, int nsteps, int k_ckpt,
                      int objective, double *grad_u0, double *grad_force, double *J, int where);
 
+/* ---- Seeded initial fields (host only, no device work). */
+
+/* initial_bc::DisturbanceEngine, which is std::mt19937_64 (src/sim/initial_bc.hpp:13), seeded
+ * as DisturbanceEngine(seed).  cdns_engine_draw writes n draws in sequence: kind 0 the raw 64-bit
+ * outputs, kind 1 initial_bc::centered_uniform, (engine() >> 11) 2^-53 - 0.5
+ * (src/sim/initial_bc.cpp:15-18), as doubles.  The engine continues where the last draw ended. */
+typedef struct cdns_engine cdns_engine;
+int cdns_engine_create(unsigned long long seed, cdns_engine **out);
+int cdns_engine_draw(cdns_engine *engine, int kind, void *out, unsigned long long n);
+void cdns_engine_destroy(cdns_engine *engine);
+
 #ifdef __cplusplus
 }
 #endif
