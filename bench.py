@@ -37,6 +37,8 @@ def algorithmic_bytes(cfg, esz):
     return {
         "curl": 6 * S * zc,
         "y_backward": 6 * S * zc + 6 * Y * esz,
+        # u, v, w in; u, v, w and the vorticity out (the curl's wall sums: 2 complex per column)
+        "y_backward_curl": 3 * S * zc + 6 * Y * esz + NQ * 2 * zc,
         "z_backward": 6 * Y * esz + 6 * Z * esz,
         "x_product": 6 * Z * esz + 3 * Z * esz,
         "z_forward": 3 * Z * esz + 3 * Y * esz,

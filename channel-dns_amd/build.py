@@ -13,6 +13,9 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs: with AGPR accumulators the y kernels copied every
+    # accumulator out and back in each k-step
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -88,6 +88,24 @@ void y_tables(int N, int nj, int NLH, std::vector<double> &B, std::vector<double
     }
 }
 
+// T'_j(y_l) at y_l = cos(pi l / N), l < NLH: j^2 at the wall, j sin(pi j l / N) / sin(pi l / N)
+// elsewhere (the derivative of T_j = cos(j theta)), from exact-angle sines.
+void y_dtable(int N, int nj, int NLH, std::vector<double> &D) {
+    const long double pi = acosl(-1.0L);
+    auto sinpi = [&](long long m) { // sin(pi m / N)
+        m %= 2 * N;
+        if (m % N == 0) return 0.0L;
+        if (2 * m == N) return 1.0L;
+        if (2 * m == 3 * N) return -1.0L;
+        return sinl(pi * (long double)m / (long double)N);
+    };
+    D.assign((size_t)nj * NLH, 0.0);
+    for (int j = 0; j < nj; ++j)
+        for (int l = 0; l < NLH; ++l)
+            D[(size_t)j * NLH + l] = l == 0 ? (double)j * (double)j
+                                            : (double)((long double)j * sinpi((long long)j * l) / sinpi(l));
+}
+
 struct Event2 {
     hipEvent_t a, b;
     int name;
@@ -105,12 +123,13 @@ struct cdns_ctx {
 
     // tables
     double *WAVX = nullptr, *WAVZ = nullptr, *WL = nullptr, *WD = nullptr, *WR = nullptr;
-    double *ycoef_b = nullptr, *ycoef_fe = nullptr, *ycoef_fo = nullptr;
+    double *ycoef_b = nullptr, *ycoef_fe = nullptr, *ycoef_fo = nullptr, *ycoef_d = nullptr;
     void *twz = nullptr, *twx = nullptr;
     FftPlan pz{}, px{};
     int KT = 13, P = 4;
     int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
     int ymfma = 1; // y stages on the matrix cores (CDNS_YSTAGE=vector selects the vector-ALU kernels)
+    int ycurl = 1; // curl fused into the matrix-core y backward of the step (CDNS_YCURL=0: separate)
     int xpair = 1; // x-stage product FFT rows on lane pairs (CDNS_XPRODUCT=single keeps one lane per row)
     int zpair = 1; // z-stage FFT rows on lane pairs (CDNS_ZSTAGE=single keeps one lane per row)
     std::vector<double> hWAVX, hWAVZ;
@@ -289,6 +308,11 @@ XArgs xargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs,
 }
 
 // spectral planes -> Y fields (n fields) -> Z fields (the context's Z buffer, or zout)
+// Y fields (n) -> Z fields (the context's Z buffer, or zout)
+void backward_z(cdns_ctx *c, int n, void *zout = nullptr) {
+    Launch L(c, "z_backward");
+    launch_zbwd(zargs(c, c->Ybuf, zout ? zout : c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
+}
 void backward_yz(cdns_ctx *c, const zd *const *planes, int n, void *zout = nullptr) {
     YArgs a = yargs(c);
     for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
@@ -296,10 +320,7 @@ void backward_yz(cdns_ctx *c, const zd *const *planes, int n, void *zout = nullp
         Launch L(c, "y_backward");
         launch_ybwd(a, n, c->fp32, c->st);
     }
-    {
-        Launch L(c, "z_backward");
-        launch_zbwd(zargs(c, c->Ybuf, zout ? zout : c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
-    }
+    backward_z(c, n, zout);
 }
 // ZN fields (n, the context's ZN buffer or zin) -> Y fields -> spectral planes (+ add planes)
 void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr, zd *const *add = nullptr) {
@@ -516,8 +537,34 @@ void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
 
 // TimeStepper::enqueue_nonlinear_term on the context's velocity planes
 void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
+    zd *const *add = c->body_on ? c->body : nullptr;
+    if (c->ymfma && c->ycurl) {
+        // y backward of u with the curl fused in: the six Y fields and the wall condition
+        YArgs a = yargs(c);
+        for (int k = 0; k < 3; ++k) a.spec[k] = u[k];
+        a.coefd = c->ycoef_d;
+        a.wavx = c->WAVX;
+        a.wavz = c->WAVZ;
+        a.Re = c->cfg.Re;
+        a.nzk = c->g.nzk;
+        a.bcw = c->bcw;
+        bool done;
+        {
+            Launch L(c, "y_backward_curl");
+            done = launch_ybwd_curl(a, c->fp32, c->st);
+        }
+        if (done) {
+            backward_z(c, 6);
+            {
+                Launch L(c, "x_product");
+                launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
+            }
+            forward_zy(c, nl, 3, nullptr, add);
+            return;
+        }
+    }
     curl_planes(c, u, c->om, c->bcw);
-    cross_planes(c, u, c->om, nl, c->body_on ? c->body : nullptr);
+    cross_planes(c, u, c->om, nl, add);
 }
 
 // ---------------------------------------------------------------- solver --
@@ -944,6 +991,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         c->fp32 = cfg->precision == CDNS_MIXED;
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
         if (const char *e = getenv("CDNS_YSTAGE")) c->ymfma = std::string(e) != "vector";
+        if (const char *e = getenv("CDNS_YCURL")) c->ycurl = std::string(e) != "0";
         if (const char *e = getenv("CDNS_XPRODUCT")) c->xpair = std::string(e) != "single";
         if (const char *e = getenv("CDNS_ZSTAGE")) c->zpair = std::string(e) != "single";
         c->esz = c->fp32 ? 8 : 16;
@@ -994,6 +1042,9 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         C->ycoef_b = upload(B);
         C->ycoef_fe = upload(Fe);
         C->ycoef_fo = upload(Fo);
+        std::vector<double> D;
+        y_dtable(g.NY2, g.nj, g.NLH, D);
+        C->ycoef_d = upload(D);
 
         // FFT plans and twiddles
         C->pz = make_plan(g.NZ2);

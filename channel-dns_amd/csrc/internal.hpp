@@ -17,6 +17,13 @@ struct YArgs {
     int mfma;                   // use the matrix-core kernels when the tables fit
     int has_mid, rows_m, njp, KL; // matrix-core shape (set by their launchers)
     int parts, ncg;               // matrix-core grid: parts per column group, column groups
+    // backward with the curl fused in (launch_ybwd_curl): spec[0..2] = u, v, w; outputs the six
+    // fields u, v, w, wx, wy, wz and (if bcw) the curl part of the pressure wall condition
+    const double *coefd;          // T'_j(y_l) [j][NLH]
+    const double *wavx, *wavz;
+    double Re;
+    int nzk;
+    zd *bcw;
 };
 
 // z stages (FFT over kz).
@@ -62,6 +69,7 @@ void launch_ybwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
 // matrix-core variants; return false when a table does not fit the LDS budget
 bool launch_ybwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st);
 bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st);
+bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st);
 void launch_yfwd(const YArgs &a, int nfields, bool fp32, hipStream_t st);
 void launch_zbwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);
 void launch_zfwd(const ZArgs &a, int nfields, bool fp32, hipStream_t st);

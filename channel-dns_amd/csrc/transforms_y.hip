@@ -50,6 +50,9 @@ template <> struct Mfma<float> {
 
 constexpr int kLdsBudget = 64 * 1024;
 constexpr int kWaves = 8;
+// fused backward: row tiles per part (five accumulator sets each) and waves per block
+constexpr int kYCurlTiles = 2;
+constexpr int kYCurlWaves64 = 4, kYCurlWaves32 = 8;
 
 // ------------------------------------------------------------------ backward --
 // grid: (column groups of 8 waves, row parts, fields); TRP row tiles (16 l' each) per part.
@@ -105,44 +108,49 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
             accO[c][t] = v4{R(0), R(0), R(0), R(0)};
         }
     }
-    // k-step s covers j = 8s .. 8s+7: this lane's even j = 8s + 2k, odd j = 8s + 2k + 1
+    // k-step s covers j = 8s .. 8s+7: this lane's even j = 8s + 2k, odd j = 8s + 2k + 1.  Every
+    // load is unconditional (column and j clamped into the plane) and j >= nj is masked by a
+    // multiply: a select or a load under a branch makes the compiler wait for every load in
+    // flight, the next step's prefetch included.  Columns past NQ compute garbage that is not
+    // stored.
     const int nsteps = a.njp / 8;
-    zd bv[CT];
-    R bE[CT], bO[CT];
-    auto load = [&](int s) {
+    struct Buf { // raw loads of one step
+        zd v[CT];
+        double e[CT], o[CT];
+    };
+    auto load = [&](Buf &b, int s) {
         const int je = 8 * s + 2 * k;
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             if constexpr (kFused) {
                 const int j = je + hh;
-                const zd t = src[c][j < nj ? j : 0];
-                bv[c] = (okq[c] && j < nj) ? t : zd{0.0, 0.0};
+                b.v[c] = src[c][j < nj ? j : 0];
             } else {
                 const int jo = je + 1;
-                bE[c] = (okq[c] && je < nj) ? R(srcp[c][2 * je]) : R(0);
-                bO[c] = (okq[c] && jo < nj) ? R(srcp[c][2 * jo]) : R(0);
+                b.e[c] = srcp[c][2 * (je < nj ? je : 0)];
+                b.o[c] = srcp[c][2 * (jo < nj ? jo : 0)];
             }
         }
     };
-    load(0);
-    for (int s = 0; s < nsteps; ++s) {
+    auto compute = [&](const Buf &b, int s) {
+        const int je = 8 * s + 2 * k;
+        const double mkE = je < nj ? 1.0 : 0.0, mkO = je + 1 < nj ? 1.0 : 0.0;
         R cE[CT], cO[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             if constexpr (kFused) {
-                const R vx = R(bv[c].x), vy = R(bv[c].y);
+                const R mk = R(hh ? mkO : mkE);
+                const R vx = R(b.v[c].x) * mk, vy = R(b.v[c].y) * mk;
                 const R px = pair_swap(vx), py = pair_swap(vy);
                 // even lanes: Re S[j even] (own), Re S[j odd] (partner); odd lanes: Im S[j even]
                 // (partner), Im S[j odd] (own)
                 cE[c] = hh ? py : vx;
                 cO[c] = hh ? vy : px;
             } else {
-                cE[c] = bE[c];
-                cO[c] = bO[c];
+                cE[c] = R(b.e[c] * mkE);
+                cO[c] = R(b.o[c] * mkO);
             }
         }
-        if (s + 1 < nsteps) load(s + 1);
-        const int je = 8 * s + 2 * k;
         const R *rowE = Bt + (size_t)je * PITCH + col, *rowO = rowE + PITCH;
 #pragma unroll
         for (int t = 0; t < TRP; ++t) {
@@ -158,6 +166,16 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
 #pragma unroll
             for (int c = 0; c < CT; ++c) mid[c] += mc * cE[c];
         }
+    };
+    // two buffers: the loads of a step land in registers the running step does not read; loads
+    // past the last step read clamped addresses and are not used
+    Buf b0, b1;
+    load(b0, 0);
+    for (int s = 0; s < nsteps; s += 2) {
+        load(b1, s + 1);
+        compute(b0, s);
+        load(b0, s + 2);
+        if (s + 1 < nsteps) compute(b1, s + 1);
     }
     R *Y = reinterpret_cast<R *>(a.y) + (size_t)f * a.YS * 2;
     const int NY2 = a.NY2;
@@ -183,6 +201,279 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
                 }
             }
     }
+}
+
+// ---------------------------------------------------- backward with the curl --
+// The velocity's y backward with vector_ops::enqueue_curl (src/solver/vector_ops.cpp:112-151)
+// and the curl part of the pressure wall condition (src/solver/timestepper.cpp:26-43) fused in.
+// The y derivative is applied as a second synthesis matrix, T'_j(y_l) (the exact derivative of
+// the Chebyshev series at the Gauss-Lobatto points, what the recurrence of vector_ops.cpp:17-25
+// followed by the synthesis gives), so the six physical fields
+//   u, v, w,  wx = w' - i kz v,  wy = i kz u - i kx w,  wz = i kx v - u'
+// come from the three velocity planes and the vorticity planes are never formed.  Per column
+// group: five matrix applies (T on u, v, w; T' on u, w).  T'_j(-y) = (-1)^(j+1) T'_j(y): the
+// derivative rows l and NY2 - l are O + E and O - E.
+// Wall condition (FP64 in both precisions, from the loaded values): with v(+-1) = sum (+-1)^j v_j
+// and u'(+-1) = sum (+-1)^(j+1) j^2 u_j,
+//   bcw(wall 0) = (-k^2 v(1) - i kx u'(1) - i kz w'(1)) / Re,  bcw(wall 1) = -(same at y = -1),
+// which is sum_j t_j and sum_j (-1)^(j+1) t_j of t_j = (i kx/Re) wz_j - (i kz/Re) wx_j.
+// grid: (column groups of W waves x 8 complex columns, row parts); TRP row tiles per part.
+template <typename R, int TRP, int W>
+__global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
+    using M = Mfma<R>;
+    using v4 = typename M::v4;
+    constexpr int PITCH = TRP * 16;
+    constexpr bool kFused = sizeof(R) == 4; // 16-byte loads + lane-pair swaps, as k_ybwd_mfma
+    extern __shared__ unsigned char smem_raw[];
+    R *Bt = reinterpret_cast<R *>(smem_raw); // [njp][PITCH] T_j(y_l)
+    R *Dt = Bt + (size_t)a.njp * PITCH;      // [njp][PITCH] T'_j(y_l)
+    R *midc = Dt + (size_t)a.njp * PITCH;    // [njp] T_j(0)
+    R *midd = midc + a.njp;                  // [njp] T'_j(0)
+    const int L = xcd_block((int)gridDim.x);
+    const int part = L % a.parts, cg = L / a.parts;
+    const int l0 = part * PITCH;
+    const bool do_mid = a.has_mid && part == a.parts - 1;
+    const bool do_wall = a.bcw != nullptr && part == 0;
+    const int NLH = a.NLH, nj = a.nj, rows = a.rows_m;
+    for (int e = threadIdx.x; e < a.njp * PITCH; e += blockDim.x) {
+        const int j = e / PITCH, lp = l0 + (e - j * PITCH);
+        const bool ok = j < nj && lp < rows;
+        Bt[e] = ok ? R(a.coef[(size_t)j * NLH + lp]) : R(0);
+        Dt[e] = ok ? R(a.coefd[(size_t)j * NLH + lp]) : R(0);
+    }
+    if (do_mid)
+        for (int j = threadIdx.x; j < a.njp; j += blockDim.x) {
+            midc[j] = j < nj ? R(a.coef[(size_t)j * NLH + NLH - 1]) : R(0);
+            midd[j] = j < nj ? R(a.coefd[(size_t)j * NLH + NLH - 1]) : R(0);
+        }
+    __syncthreads();
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, k = lane >> 4, hh = col & 1;
+    const int q = (cg * W + wv) * 8 + (col >> 1);
+    const bool okq = q < a.NQ;
+    const size_t qoff = (size_t)(okq ? q : 0) * a.JS;
+    const zd *src[3];
+    const double *srcp[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        src[f] = a.spec[f] + qoff;
+        srcp[f] = reinterpret_cast<const double *>(a.spec[f]) + qoff * 2 + hh;
+    }
+    // accumulators: 0 U, 1 V, 2 W (T), 3 U', 4 W' (T')
+    v4 accE[5][TRP], accO[5][TRP];
+    R mid[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        mid[f] = R(0);
+#pragma unroll
+        for (int t = 0; t < TRP; ++t) {
+            accE[f][t] = v4{R(0), R(0), R(0), R(0)};
+            accO[f][t] = v4{R(0), R(0), R(0), R(0)};
+        }
+    }
+    // wall sums v(1), v(-1), u'(1), u'(-1), w'(1), w'(-1): FP64 own part (FP64 kernel) or
+    // complex (FP32 kernel, which holds whole complex values)
+    double ws[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    zd wc[6] = {};
+    const int nsteps = a.njp / 8;
+    // two load buffers, so that the loads of step s + 1 land in registers that step s does not
+    // read (no copies that would wait for them)
+    struct Buf { // raw loads; j >= nj is masked where they are used
+        zd v[3];
+        double e[3], o[3];
+    };
+    auto load = [&](Buf &b, int s) {
+        const int je = 8 * s + 2 * k;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if constexpr (kFused) {
+                const int j = je + hh;
+                b.v[f] = src[f][j < nj ? j : 0];
+            } else {
+                const int jo = je + 1;
+                b.e[f] = srcp[f][2 * (je < nj ? je : 0)];
+                b.o[f] = srcp[f][2 * (jo < nj ? jo : 0)];
+            }
+        }
+    };
+    auto compute = [&](const Buf &b, int s) {
+        const int je = 8 * s + 2 * k;
+        const double mkE = je < nj ? 1.0 : 0.0, mkO = je + 1 < nj ? 1.0 : 0.0;
+        zd v[3];
+        double e[3], o[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if constexpr (kFused) {
+                const double mk = hh ? mkO : mkE;
+                v[f] = {b.v[f].x * mk, b.v[f].y * mk};
+            } else {
+                e[f] = b.e[f] * mkE;
+                o[f] = b.o[f] * mkO;
+            }
+        }
+        if (do_wall) {
+            if constexpr (kFused) {
+                const int j = je + hh;
+                const double j2 = double(j) * double(j), sj = hh ? -1.0 : 1.0; // (-1)^j
+                wc[0] = wc[0] + v[1];
+                wc[1] = wc[1] + sj * v[1];
+                wc[2] = wc[2] + j2 * v[0];
+                wc[3] = wc[3] + (-sj * j2) * v[0];
+                wc[4] = wc[4] + j2 * v[2];
+                wc[5] = wc[5] + (-sj * j2) * v[2];
+            } else {
+                const double e2 = double(je) * double(je), o2 = double(je + 1) * double(je + 1);
+                ws[0] += e[1] + o[1];
+                ws[1] += e[1] - o[1];
+                ws[2] += e2 * e[0] + o2 * o[0];
+                ws[3] += o2 * o[0] - e2 * e[0];
+                ws[4] += e2 * e[2] + o2 * o[2];
+                ws[5] += o2 * o[2] - e2 * e[2];
+            }
+        }
+        R cE[3], cO[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if constexpr (kFused) {
+                const R vx = R(v[f].x), vy = R(v[f].y);
+                const R px = pair_swap(vx), py = pair_swap(vy);
+                cE[f] = hh ? py : vx;
+                cO[f] = hh ? vy : px;
+            } else {
+                cE[f] = R(e[f]);
+                cO[f] = R(o[f]);
+            }
+        }
+        const R *rowE = Bt + (size_t)je * PITCH + col, *rowO = rowE + PITCH;
+        const R *drE = Dt + (size_t)je * PITCH + col, *drO = drE + PITCH;
+#pragma unroll
+        for (int t = 0; t < TRP; ++t) {
+            const R aE = rowE[16 * t], aO = rowO[16 * t], dE = drE[16 * t], dO = drO[16 * t];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                accE[f][t] = M::mma(aE, cE[f], accE[f][t]);
+                accO[f][t] = M::mma(aO, cO[f], accO[f][t]);
+            }
+            accE[3][t] = M::mma(dE, cE[0], accE[3][t]);
+            accO[3][t] = M::mma(dO, cO[0], accO[3][t]);
+            accE[4][t] = M::mma(dE, cE[2], accE[4][t]);
+            accO[4][t] = M::mma(dO, cO[2], accO[4][t]);
+        }
+        if (do_mid) {
+            // T_j(0) vanishes for odd j, T'_j(0) for even j
+            const R mc = midc[je], md = midd[je + 1];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) mid[f] += mc * cE[f];
+            mid[3] += md * cO[0];
+            mid[4] += md * cO[2];
+        }
+    };
+    // loads past the last step read clamped (in-plane) addresses and are not used: no branch
+    // around a load, so the compiler counts the loads in flight exactly
+    Buf b0, b1;
+    load(b0, 0);
+    for (int s = 0; s < nsteps; s += 2) {
+        load(b1, s + 1);
+        compute(b0, s);
+        load(b0, s + 2);
+        if (s + 1 < nsteps) compute(b1, s + 1);
+    }
+
+    const int kxi = okq ? q / a.nzk : 0, kzi = okq ? q - kxi * a.nzk : 0;
+    const double kxd = a.wavx[kxi], kzd = a.wavz[kzi];
+    if (do_wall) {
+        zd sum[6];
+        if constexpr (kFused) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                zd v = wc[i];
+                v.x += __shfl_xor(v.x, 16);
+                v.y += __shfl_xor(v.y, 16);
+                v.x += __shfl_xor(v.x, 32);
+                v.y += __shfl_xor(v.y, 32);
+                v.x += __shfl_xor(v.x, 1);
+                v.y += __shfl_xor(v.y, 1);
+                sum[i] = v;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double v = ws[i];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                const double o = __shfl_xor(v, 1);
+                sum[i] = hh ? zd{o, v} : zd{v, o};
+            }
+        }
+        if (okq && k == 0 && hh == 0) {
+            const double k2 = kxd * kxd + kzd * kzd, inv = 1.0 / a.Re;
+            // -k^2 v - i kx u' - i kz w'
+            const zd wp = {-k2 * sum[0].x + kxd * sum[2].y + kzd * sum[4].y,
+                           -k2 * sum[0].y - kxd * sum[2].x - kzd * sum[4].x};
+            const zd wm = {-k2 * sum[1].x + kxd * sum[3].y + kzd * sum[5].y,
+                           -k2 * sum[1].y - kxd * sum[3].x - kzd * sum[5].x};
+            a.bcw[(size_t)q * 2 + 0] = {wp.x * inv, wp.y * inv};
+            a.bcw[(size_t)q * 2 + 1] = {-wm.x * inv, -wm.y * inv};
+        }
+    }
+
+    // i k X in this lane's part: sg k (X's other part), which the pair partner holds
+    const R kx = R(kxd), kz = R(kzd), sg = hh ? R(1) : R(-1);
+    R *Y = reinterpret_cast<R *>(a.y);
+    const size_t FS = a.YS * 2;
+    const int NY2 = a.NY2;
+    if (do_mid) {
+        R m[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            R v = mid[f];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            m[f] = v;
+        }
+        const R su = pair_swap(m[0]), sv = pair_swap(m[1]), sw = pair_swap(m[2]);
+        const R out[6] = {m[0], m[1], m[2], m[4] - sg * kz * sv, sg * (kz * su - kx * sw), sg * kx * sv - m[3]};
+        if (k == 0 && okq) {
+            const size_t o = ((size_t)(NLH - 1) * a.NQ + q) * 2 + hh;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) Y[f * FS + o] = out[f];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TRP; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int lp = l0 + 16 * t + M::row(k, r);
+            R up[3], um[3];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                up[f] = accE[f][t][r] + accO[f][t][r];
+                um[f] = accE[f][t][r] - accO[f][t][r];
+            }
+            const R dup = accO[3][t][r] + accE[3][t][r], dum = accO[3][t][r] - accE[3][t][r];
+            const R dwp = accO[4][t][r] + accE[4][t][r], dwm = accO[4][t][r] - accE[4][t][r];
+            R sp[3], sm[3];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                sp[f] = pair_swap(up[f]);
+                sm[f] = pair_swap(um[f]);
+            }
+            if (okq && lp < rows) {
+                const size_t op = ((size_t)lp * a.NQ + q) * 2 + hh, om = ((size_t)(NY2 - lp) * a.NQ + q) * 2 + hh;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    Y[f * FS + op] = up[f];
+                    Y[f * FS + om] = um[f];
+                }
+                Y[3 * FS + op] = dwp - sg * kz * sp[1];
+                Y[3 * FS + om] = dwm - sg * kz * sm[1];
+                Y[4 * FS + op] = sg * (kz * sp[0] - kx * sp[2]);
+                Y[4 * FS + om] = sg * (kz * sm[0] - kx * sm[2]);
+                Y[5 * FS + op] = sg * kx * sp[1] - dup;
+                Y[5 * FS + om] = sg * kx * sm[1] - dum;
+            }
+        }
 }
 
 // ------------------------------------------------------------------- forward --
@@ -354,6 +645,31 @@ bool launch_ybwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
         case 3: go_ybwd<double, 3, 1>(a, parts, nf, st); break;
         default: go_ybwd<double, 4, 1>(a, parts, nf, st); break;
         }
+    }
+    return true;
+}
+
+template <typename R, int TRP, int W> static void go_ybwd_curl(YArgs a, int parts, hipStream_t st) {
+    const size_t sm = ((size_t)2 * a.njp * TRP * 16 + 2 * a.njp) * sizeof(R);
+    a.parts = parts;
+    a.ncg = (a.NQ + 8 * W - 1) / (8 * W);
+    hipLaunchKernelGGL((k_ybwd_curl<R, TRP, W>), dim3(a.parts * a.ncg), dim3(64 * W), sm, st, a);
+}
+
+bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
+    y_shape(a);
+    const size_t esz = fp32 ? 4 : 8;
+    if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * 16 * esz + 2 * a.njp * esz > (size_t)kLdsBudget) return false;
+    const int TR = (a.rows_m + 15) / 16;
+    const int fit = std::min(tiles_per_part((size_t)2 * a.njp * 16 * esz, 2 * a.njp * esz, kYCurlTiles), TR);
+    const int parts = (TR + fit - 1) / fit;
+    const int trp = (TR + parts - 1) / parts;
+    if (fp32) {
+        if (trp == 1) go_ybwd_curl<float, 1, kYCurlWaves32>(a, parts, st);
+        else go_ybwd_curl<float, 2, kYCurlWaves32>(a, parts, st);
+    } else {
+        if (trp == 1) go_ybwd_curl<double, 1, kYCurlWaves64>(a, parts, st);
+        else go_ybwd_curl<double, 2, kYCurlWaves64>(a, parts, st);
     }
     return true;
 }

@@ -23,14 +23,17 @@ CONFIGS = {
 
 
 # Two kernel families per stage: the defaults (register-resident two-pass FFTs for x and z,
-# matrix-core y stages) and the alternates (LDS Stockham FFTs, CDNS_FFT=stockham; vector-ALU
-# y stages, CDNS_YSTAGE=vector).
-FFT_KINDS = ["register", "stockham", "single-lane-rows"]
+# matrix-core y stages with the curl fused into the y backward) and the alternates (LDS Stockham
+# FFTs, CDNS_FFT=stockham; vector-ALU y stages, CDNS_YSTAGE=vector; the curl as its own kernel
+# before the matrix-core y backward, CDNS_YCURL=0).
+FFT_KINDS = ["register", "stockham", "single-lane-rows", "separate-curl"]
 
 
 def _use_fft(monkeypatch, kind):
-    for v in ("CDNS_FFT", "CDNS_YSTAGE", "CDNS_XPRODUCT", "CDNS_ZSTAGE", "CDNS_SOLVER_TABLES"):
+    for v in ("CDNS_FFT", "CDNS_YSTAGE", "CDNS_XPRODUCT", "CDNS_ZSTAGE", "CDNS_SOLVER_TABLES", "CDNS_YCURL"):
         monkeypatch.delenv(v, raising=False)
+    if kind == "separate-curl":
+        monkeypatch.setenv("CDNS_YCURL", "0")
     if kind == "stockham":
         monkeypatch.setenv("CDNS_FFT", "stockham")
         monkeypatch.setenv("CDNS_YSTAGE", "vector")
@@ -138,7 +141,8 @@ def test_solve_modes(gpu, correct, fuse):
                                                 ("stage3", 5, 1e-10, "register"),
                                                 ("channel_32", 5, 1e-10, "stockham"),
                                                 ("stage3", 5, 1e-10, "stockham"),
-                                                ("channel_32", 5, 1e-10, "single-lane-rows")])
+                                                ("channel_32", 5, 1e-10, "single-lane-rows"),
+                                                ("channel_32", 5, 1e-10, "separate-curl")])
 def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
     _use_fft(monkeypatch, fft)
     cfg = stage3_config() if name == "stage3" else CONFIGS[name]
