@@ -50,9 +50,36 @@ template <> struct Mfma<float> {
 
 constexpr int kLdsBudget = 64 * 1024;
 constexpr int kWaves = 8;
-// fused backward: row tiles per part (five accumulator sets each) and waves per block
+
+// LDS table fill: dst[e] = R(src[i]) where idx(e, i) holds, else 0, for e < n.  Eight loads per
+// thread are issued before any is used (unconditional, at a clamped index), so a block's fill
+// costs about one global-memory latency per eight rows instead of one per row.
+template <typename R, typename Idx>
+__device__ inline void fill_lds(R *dst, const double *src, int n, Idx idx) {
+    constexpr int U = 8;
+    for (int e0 = threadIdx.x; e0 < n; e0 += U * (int)blockDim.x) {
+        double v[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * (int)blockDim.x;
+            int i = 0;
+            ok[u] = e < n && idx(e, i);
+            v[u] = src[ok[u] ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * (int)blockDim.x;
+            if (e < n) dst[e] = ok[u] ? R(v[u]) : R(0);
+        }
+    }
+}
+// fused backward: row tiles per part (five accumulator sets each), waves per block, steps the
+// loads run ahead, LDS budget of its two tables
 constexpr int kYCurlTiles = 2;
 constexpr int kYCurlWaves64 = 4, kYCurlWaves32 = 8;
+constexpr int kYCurlDepth = 1;
+constexpr int kYCurlLds = 64 * 1024;
 
 // ------------------------------------------------------------------ backward --
 // grid: (column groups of 8 waves, row parts, fields); TRP row tiles (16 l' each) per part.
@@ -70,12 +97,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_ybwd_mfma(YArgs a) {
     const int l0 = part * PITCH;
     const bool do_mid = a.has_mid && part == a.parts - 1;
     const int NLH = a.NLH, nj = a.nj, rows = a.rows_m;
-    for (int e = threadIdx.x; e < a.njp * PITCH; e += blockDim.x) {
+    auto tidx = [&](int e, int &i) {
         const int j = e / PITCH, lp = l0 + (e - j * PITCH);
-        Bt[e] = (j < nj && lp < rows) ? R(a.coef[(size_t)j * NLH + lp]) : R(0);
-    }
+        i = j * NLH + lp;
+        return j < nj && lp < rows;
+    };
+    fill_lds(Bt, a.coef, a.njp * PITCH, tidx);
     if (do_mid)
-        for (int j = threadIdx.x; j < a.njp; j += blockDim.x) midc[j] = j < nj ? R(a.coef[(size_t)j * NLH + NLH - 1]) : R(0);
+        fill_lds(midc, a.coef, a.njp, [&](int j, int &i) {
+            i = j * NLH + NLH - 1;
+            return j < nj;
+        });
     __syncthreads();
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -235,17 +267,21 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
     const bool do_mid = a.has_mid && part == a.parts - 1;
     const bool do_wall = a.bcw != nullptr && part == 0;
     const int NLH = a.NLH, nj = a.nj, rows = a.rows_m;
-    for (int e = threadIdx.x; e < a.njp * PITCH; e += blockDim.x) {
+    auto tidx = [&](int e, int &i) {
         const int j = e / PITCH, lp = l0 + (e - j * PITCH);
-        const bool ok = j < nj && lp < rows;
-        Bt[e] = ok ? R(a.coef[(size_t)j * NLH + lp]) : R(0);
-        Dt[e] = ok ? R(a.coefd[(size_t)j * NLH + lp]) : R(0);
+        i = j * NLH + lp;
+        return j < nj && lp < rows;
+    };
+    auto midx = [&](int j, int &i) {
+        i = j * NLH + NLH - 1;
+        return j < nj;
+    };
+    fill_lds(Bt, a.coef, a.njp * PITCH, tidx);
+    fill_lds(Dt, a.coefd, a.njp * PITCH, tidx);
+    if (do_mid) {
+        fill_lds(midc, a.coef, a.njp, midx);
+        fill_lds(midd, a.coefd, a.njp, midx);
     }
-    if (do_mid)
-        for (int j = threadIdx.x; j < a.njp; j += blockDim.x) {
-            midc[j] = j < nj ? R(a.coef[(size_t)j * NLH + NLH - 1]) : R(0);
-            midd[j] = j < nj ? R(a.coefd[(size_t)j * NLH + NLH - 1]) : R(0);
-        }
     __syncthreads();
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -371,13 +407,28 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
     };
     // loads past the last step read clamped (in-plane) addresses and are not used: no branch
     // around a load, so the compiler counts the loads in flight exactly
-    Buf b0, b1;
-    load(b0, 0);
-    for (int s = 0; s < nsteps; s += 2) {
-        load(b1, s + 1);
-        compute(b0, s);
-        load(b0, s + 2);
-        if (s + 1 < nsteps) compute(b1, s + 1);
+    if constexpr (kYCurlDepth == 2) {
+        // three buffers: the loads run two steps ahead
+        Buf b0, b1, b2;
+        load(b0, 0);
+        load(b1, 1);
+        for (int s = 0; s < nsteps; s += 3) {
+            load(b2, s + 2);
+            compute(b0, s);
+            load(b0, s + 3);
+            if (s + 1 < nsteps) compute(b1, s + 1);
+            load(b1, s + 4);
+            if (s + 2 < nsteps) compute(b2, s + 2);
+        }
+    } else {
+        Buf b0, b1;
+        load(b0, 0);
+        for (int s = 0; s < nsteps; s += 2) {
+            load(b1, s + 1);
+            compute(b0, s);
+            load(b0, s + 2);
+            if (s + 1 < nsteps) compute(b1, s + 1);
+        }
     }
 
     const int kxi = okq ? q / a.nzk : 0, kzi = okq ? q - kxi * a.nzk : 0;
@@ -491,15 +542,21 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
     const int part = L % a.parts, cgf = L / a.parts, cg = cgf % a.ncg, f = cgf / a.ncg;
     const int m0 = part * PITCH;
     const int rows = a.rows_m, nje = a.nje, njo = a.njo, NLH = a.NLH;
-    for (int e = threadIdx.x; e < a.KL * PITCH; e += blockDim.x) {
+    fill_lds(Fe, a.coef, a.KL * PITCH, [&](int e, int &i) {
         const int lp = e / PITCH, m = m0 + (e - lp * PITCH);
-        Fe[e] = (lp < rows && m < nje) ? R(a.coef[(size_t)lp * nje + m]) : R(0);
-        Fo[e] = (lp < rows && m < njo) ? R(a.coef2[(size_t)lp * njo + m]) : R(0);
-    }
-    for (int e = threadIdx.x; e < PITCH; e += blockDim.x) {
+        i = lp * nje + m;
+        return lp < rows && m < nje;
+    });
+    fill_lds(Fo, a.coef2, a.KL * PITCH, [&](int e, int &i) {
+        const int lp = e / PITCH, m = m0 + (e - lp * PITCH);
+        i = lp * njo + m;
+        return lp < rows && m < njo;
+    });
+    fill_lds(Fm, a.coef, PITCH, [&](int e, int &i) {
         const int m = m0 + e;
-        Fm[e] = (a.has_mid && m < nje) ? R(a.coef[(size_t)(NLH - 1) * nje + m]) : R(0);
-    }
+        i = (NLH - 1) * nje + m;
+        return a.has_mid && m < nje;
+    });
     __syncthreads();
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -653,15 +710,24 @@ template <typename R, int TRP, int W> static void go_ybwd_curl(YArgs a, int part
     const size_t sm = ((size_t)2 * a.njp * TRP * 16 + 2 * a.njp) * sizeof(R);
     a.parts = parts;
     a.ncg = (a.NQ + 8 * W - 1) / (8 * W);
+    if (sm > 64 * 1024) { // above the default dynamic LDS limit of a launch (the CU has 160 KB)
+        static bool raised = false;
+        if (!raised) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ybwd_curl<R, TRP, W>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+            raised = true;
+        }
+    }
     hipLaunchKernelGGL((k_ybwd_curl<R, TRP, W>), dim3(a.parts * a.ncg), dim3(64 * W), sm, st, a);
 }
 
 bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
     y_shape(a);
     const size_t esz = fp32 ? 4 : 8;
-    if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * 16 * esz + 2 * a.njp * esz > (size_t)kLdsBudget) return false;
+    if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * 16 * esz + 2 * a.njp * esz > (size_t)kYCurlLds) return false;
     const int TR = (a.rows_m + 15) / 16;
-    const int fit = std::min(tiles_per_part((size_t)2 * a.njp * 16 * esz, 2 * a.njp * esz, kYCurlTiles), TR);
+    int fit = std::min(kYCurlTiles, TR);
+    while (fit > 1 && (size_t)2 * a.njp * esz * (16 * fit + 1) > (size_t)kYCurlLds) --fit;
     const int parts = (TR + fit - 1) / fit;
     const int trp = (TR + parts - 1) / parts;
     if (fp32) {
