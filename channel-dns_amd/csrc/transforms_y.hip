@@ -80,6 +80,8 @@ constexpr int kYCurlTiles = 2;
 constexpr int kYCurlWaves64 = 4, kYCurlWaves32 = 8;
 constexpr int kYCurlDepth = 1;
 constexpr int kYCurlLds = 64 * 1024;
+// forward: waves per block (the parts' tables are shared by all of a block's column groups)
+constexpr int kYFwdWaves = 8;
 
 // ------------------------------------------------------------------ backward --
 // grid: (column groups of 8 waves, row parts, fields); TRP row tiles (16 l' each) per part.
@@ -528,9 +530,11 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
 }
 
 // ------------------------------------------------------------------- forward --
-// grid: (column groups of 8 waves, j parts, fields); JTP j tiles (16 m each) per parity per part.
-template <typename R, int JTP, int CT>
-__global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
+// grid: (column groups of W waves, j parts, fields); JTP j tiles (16 m each) per parity per part.
+// (W = 16 puts the two parts of the Re_tau=180 grid in one round of blocks instead of 1.02
+// rounds of 8-wave blocks, but measures 3 us slower in FP64 and equal in FP32.)
+template <typename R, int JTP, int CT, int W>
+__global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
     using M = Mfma<R>;
     using v4 = typename M::v4;
     constexpr int PITCH = JTP * 16;
@@ -541,7 +545,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
     const int L = xcd_block((int)gridDim.x);
     const int part = L % a.parts, cgf = L / a.parts, cg = cgf % a.ncg, f = cgf / a.ncg;
     const int m0 = part * PITCH;
-    const int rows = a.rows_m, nje = a.nje, njo = a.njo, NLH = a.NLH;
+    const int rows = a.rows_m, NLH = a.NLH, nje = a.nje, njo = a.njo;
     fill_lds(Fe, a.coef, a.KL * PITCH, [&](int e, int &i) {
         const int lp = e / PITCH, m = m0 + (e - lp * PITCH);
         i = lp * nje + m;
@@ -561,7 +565,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_yfwd_mfma(YArgs a) {
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, k = lane >> 4;
-    const int qb = (cg * kWaves + wv) * 8 * CT;
+    const int qb = (cg * W + wv) * 8 * CT;
     const R *Y = reinterpret_cast<const R *>(a.y) + (size_t)f * a.YS * 2;
     const int NQ = a.NQ, NY2 = a.NY2;
     int qa[CT];
@@ -659,11 +663,12 @@ template <typename R, int TRP, int CT> static void go_ybwd(YArgs a, int parts, i
     hipLaunchKernelGGL((k_ybwd_mfma<R, TRP, CT>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * kWaves), sm, st, a);
 }
 template <typename R, int JTP, int CT> static void go_yfwd(YArgs a, int parts, int nf, hipStream_t st) {
+    constexpr int W = kYFwdWaves;
     const size_t sm = ((size_t)2 * a.KL * JTP * 16 + JTP * 16) * sizeof(R);
-    const int cols = 8 * CT * kWaves;
+    const int cols = 8 * CT * W;
     a.parts = parts;
     a.ncg = (a.NQ + cols - 1) / cols;
-    hipLaunchKernelGGL((k_yfwd_mfma<R, JTP, CT>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * kWaves), sm, st, a);
+    hipLaunchKernelGGL((k_yfwd_mfma<R, JTP, CT, W>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * W), sm, st, a);
 }
 
 static void y_shape(YArgs &a) {
