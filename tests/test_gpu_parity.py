@@ -20,6 +20,8 @@ CONFIGS = {
     "channel_32": config.channel_32(),
     "channel_64": config.channel_64(),
 }
+# an odd product-grid y count: no middle Gauss-Lobatto point (y = 0) in the y stages
+ODD_NY2 = config.make(32, 32, 32, 16, 30, 16, 48, 47, 48, 4 * math.pi, 2 * math.pi, 180.0, 1e-3)
 
 
 # Two kernel families per stage: the defaults (register-resident two-pass FFTs for x and z,
@@ -142,10 +144,13 @@ def test_solve_modes(gpu, correct, fuse):
                                                 ("channel_32", 5, 1e-10, "stockham"),
                                                 ("stage3", 5, 1e-10, "stockham"),
                                                 ("channel_32", 5, 1e-10, "single-lane-rows"),
-                                                ("channel_32", 5, 1e-10, "separate-curl")])
+                                                ("channel_32", 5, 1e-10, "separate-curl"),
+                                                ("odd_ny2", 5, 1e-10, "register"),
+                                                ("odd_ny2", 5, 1e-10, "separate-curl"),
+                                                ("odd_ny2", 5, 1e-10, "stockham")])
 def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
     _use_fft(monkeypatch, fft)
-    cfg = stage3_config() if name == "stage3" else CONFIGS[name]
+    cfg = stage3_config() if name == "stage3" else (ODD_NY2 if name == "odd_ny2" else CONFIGS[name])
     u0 = fields.seeded_velocity(cfg, seed=5)
     dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
     dev.set_state("velocity", u0)
