@@ -545,7 +545,10 @@ template <typename R, int N1, int N2> static void go_zfwd(const ZArgs &a, int nf
     hipLaunchKernelGGL((k_zfwd_rf<R, N1, N2>), dim3(b.nkxg * a.NL * nf), dim3(64), 0, st, b);
 }
 
-// Supported lengths: N = N1 * N2 with N1, N2 <= 16.
+// Supported lengths: N = N1 * N2 with N1, N2 <= 16; the z stages also take the longer lines
+// below (factors up to 24: one line per lane group of 32, or per wavefront on lane pairs), which
+// the x product cannot hold (its three physical lines would need 288 VGPRs per lane).
+#define CDNS_RF_ZONLY_SIZES(X) X(288, 16, 18) X(384, 16, 24)
 #define CDNS_RF_SIZES(X) X(196, 14, 14) X(48, 6, 8) X(96, 8, 12) X(32, 4, 8) X(64, 8, 8) X(128, 8, 16) \
     X(144, 12, 12) X(192, 12, 16) X(256, 16, 16) X(36, 6, 6) X(72, 8, 9) X(120, 10, 12) X(160, 10, 16)
 
@@ -577,13 +580,13 @@ bool launch_xfwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st) {
 bool launch_zbwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st) {
     const int n = a.plan.n;
 #define X(NN, A, B) case NN: if (fp32) go_zbwd<float, A, B>(a, nf, st); else go_zbwd<double, A, B>(a, nf, st); return true;
-    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+    switch (n) { CDNS_RF_SIZES(X) CDNS_RF_ZONLY_SIZES(X) default: return false; }
 #undef X
 }
 bool launch_zfwd_rf(const ZArgs &a, int nf, bool fp32, hipStream_t st) {
     const int n = a.plan.n;
 #define X(NN, A, B) case NN: if (fp32) go_zfwd<float, A, B>(a, nf, st); else go_zfwd<double, A, B>(a, nf, st); return true;
-    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+    switch (n) { CDNS_RF_SIZES(X) CDNS_RF_ZONLY_SIZES(X) default: return false; }
 #undef X
 }
 
