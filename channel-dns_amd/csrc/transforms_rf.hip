@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     constexpr int W = kXWaves<R>;
     constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, LB = Gm::LB, H1 = N1 / 2, H2 = N2 / 2;
     constexpr int PB = W * NG; // points per block
-    constexpr int KP = (128 + G - 1) / G;
+    constexpr int KP = (N / 2 + G) / G; // retained kx <= K <= N/2: entries k = r + G i, i < KP
     __shared__ cpx<R> tw[N];
     __shared__ cpx<R> buf[PB][LB];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
@@ -562,7 +562,7 @@ bool rf_supported(int n) {
 bool launch_xprod_rf(const XArgs &a, bool fp32, hipStream_t st) {
     const int n = a.plan.n;
 #define X(NN, A, B) case NN: if (fp32) go_xprod<float, A, B>(a, st); else go_xprod<double, A, B>(a, st); return true;
-    switch (n) { CDNS_RF_SIZES(X) default: return false; }
+    switch (n) { CDNS_RF_SIZES(X) X(384, 16, 24) default: return false; }
 #undef X
 }
 bool launch_xbwd_phys_rf(const XArgs &a, bool fp32, hipStream_t st) {
