@@ -757,8 +757,10 @@ bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
         if (jtp == 1) go_yfwd<float, 1, 2>(a, parts, nf, st);
         else go_yfwd<float, 2, 2>(a, parts, nf, st);
     } else {
-        if (jtp == 1) go_yfwd<double, 1, 1>(a, parts, nf, st);
-        else go_yfwd<double, 2, 1>(a, parts, nf, st);
+        // two column sets per wavefront: each table read feeds four MFMAs, and the Re_tau=180
+        // grid runs in half a round of wavefronts instead of 1.02 rounds
+        if (jtp == 1) go_yfwd<double, 1, 2>(a, parts, nf, st);
+        else go_yfwd<double, 2, 2>(a, parts, nf, st);
     }
     return true;
 }
