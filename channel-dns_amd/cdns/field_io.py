@@ -31,6 +31,11 @@ FORMATS = ("ascii", "binary", "both")
 _DEC = re.compile(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?\Z")
 _HEX = re.compile(r"[+-]?0[xX]([0-9a-fA-F]+\.?[0-9a-fA-F]*|\.[0-9a-fA-F]+)([pP][+-]?\d+)?\Z")
 _WS = " \t\n\v\f\r"  # the C locale's isspace
+# one payload entry of the fast path: optional comma, "(re,im)" with decimal tokens (d/D exponents
+# allowed), whitespace as the stream extraction of the reference skips it
+_DTOK = r"[+-]?(?:\d+\.?\d*|\.\d+)(?:[eEdD][+-]?\d+)?"
+_ENTRY = re.compile(r"[ \t\n\v\f\r]*,?[ \t\n\v\f\r]*\([ \t\n\v\f\r]*(" + _DTOK + r")[ \t\n\v\f\r]*,"
+                    r"[ \t\n\v\f\r]*(" + _DTOK + r")[ \t\n\v\f\r]*\)")
 _NONFINITE = re.compile(r"[+-]?(inf|infinity|nan(\([0-9A-Za-z_]*\))?)\Z", re.IGNORECASE)
 
 
@@ -199,6 +204,9 @@ class FieldIO:
             pos += 1
         time = number()
         n = int(np.prod(self.velocity_shape()))
+        fast = self._read_entries_fast(text, pos, n)
+        if fast is not None:
+            return ntpit, time, fast
         vals = np.empty(2 * n)
         for i in range(n):
             skip_ws()
@@ -212,6 +220,27 @@ class FieldIO:
         if text[pos:].strip(_WS):
             raise RuntimeError("extra text restart data")
         return ntpit, time, vals.view(np.complex128).reshape(self.velocity_shape()).copy()
+
+    def _read_entries_fast(self, text, pos, n):
+        """The payload when every entry is a plain decimal "(re,im)" (what the writers produce),
+        parsed by one regular expression per entry and one vectorised conversion; None when the
+        text needs the general parser (hexadecimal or non-finite tokens, or any error, which the
+        general parser then reports)."""
+        toks = []
+        m = None
+        for _ in range(n):
+            m = _ENTRY.match(text, pos)
+            if m is None:
+                return None
+            toks.append(m.group(1))
+            toks.append(m.group(2))
+            pos = m.end()
+        if text[pos:].strip(_WS):
+            return None
+        vals = np.array([t.replace("d", "e").replace("D", "e") for t in toks], dtype=np.float64)
+        if not np.isfinite(vals).all():
+            return None
+        return vals.view(np.complex128).reshape(self.velocity_shape()).copy()
 
     # --------------------------------------------------------------- physical --
     def to_physical(self, velocity, pressure, finite=False):
