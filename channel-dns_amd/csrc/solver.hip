@@ -128,7 +128,9 @@ __device__ void tau_solve(const zd (&r)[NR][NE], zd (&out)[NR][NE], const double
 template <int NE, int MODE>
 __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
     const int lane = lane_id(), p = sp::parity(lane);
-    const int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the column index is wavefront-uniform: made visible as such, the per-column scalars
+    // (wavenumbers, wall values, influence and sigma entries) come through scalar loads
+    const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (q >= a.NQ) return;
     const int ny = a.ny;
     constexpr int mode = MODE;
