@@ -586,26 +586,32 @@ __global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
         }
     // k-step s: l' = 4s + k
     const int nsteps = a.KL / 4;
-    R y1[CT], y2[CT];
-    auto load = [&](int s) {
-        const int lp = 4 * s + k;
+    // loads are unconditional, so that the compiler counts the loads in flight exactly and no
+    // select waits for them: a row l' >= rows reads row rows - 1 (its table rows are zero) and a
+    // column q >= NQ reads column 0 (its outputs are not written)
+    size_t o[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) o[c] = (size_t)(oka[c] ? qa[c] : 0) * 2 + pa;
+    struct Buf {
+        R y1[CT], y2[CT];
+    };
+    auto load = [&](Buf &b, int s) {
+        int lp = 4 * s + k;
+        lp = lp < rows ? lp : rows - 1;
+        const R *r1 = Y + (size_t)lp * NQ * 2, *r2 = Y + (size_t)(NY2 - lp) * NQ * 2;
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            const bool ok = oka[c] && lp < rows;
-            const size_t o = (size_t)qa[c] * 2 + pa;
-            y1[c] = ok ? Y[(size_t)lp * NQ * 2 + o] : R(0);
-            y2[c] = ok ? Y[(size_t)(NY2 - lp) * NQ * 2 + o] : R(0);
+            b.y1[c] = r1[o[c]];
+            b.y2[c] = r2[o[c]];
         }
     };
-    load(0);
-    for (int s = 0; s < nsteps; ++s) {
+    auto compute = [&](const Buf &b, int s) {
         R sv[CT], dv[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            sv[c] = y1[c] + y2[c];
-            dv[c] = y1[c] - y2[c];
+            sv[c] = b.y1[c] + b.y2[c];
+            dv[c] = b.y1[c] - b.y2[c];
         }
-        if (s + 1 < nsteps) load(s + 1);
         const int lp = 4 * s + k;
         const R *rowE = Fe + (size_t)lp * PITCH + col, *rowO = Fo + (size_t)lp * PITCH + col;
 #pragma unroll
@@ -617,6 +623,17 @@ __global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
                 accO[c][t] = M::mma(dv[c], bo, accO[c][t]);
             }
         }
+    };
+    // two load buffers: the loads of step s + 1 land in registers that step s does not read; the
+    // step count is even (KL is a multiple of 8), so both computes are unconditional and the
+    // compiler cannot sink a buffer's loads to its use
+    Buf b0, b1;
+    load(b0, 0);
+    for (int s = 0; s < nsteps; s += 2) {
+        load(b1, s + 1);
+        compute(b0, s);
+        load(b0, s + 2);
+        compute(b1, s + 1);
     }
     // this lane holds (re, im) of complex column qout(k, h) in registers (2h, 2h + 1), for
     // m = m0 + 16t + col
@@ -675,7 +692,8 @@ static void y_shape(YArgs &a) {
     a.has_mid = (a.NY2 % 2 == 0) ? 1 : 0;
     a.rows_m = a.NLH - a.has_mid;
     a.njp = (a.nj + 7) / 8 * 8;
-    a.KL = (a.rows_m + 3) / 4 * 4;
+    // forward table rows: a multiple of 8, so that the forward kernel's k-steps (4 rows each) pair up
+    a.KL = (a.rows_m + 7) / 8 * 8;
 }
 
 // largest tile count per part, at most `cap`, whose table fits the LDS budget
