@@ -624,16 +624,25 @@ __global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
             }
         }
     };
-    // two load buffers: the loads of step s + 1 land in registers that step s does not read; the
-    // step count is even (KL is a multiple of 8), so both computes are unconditional and the
-    // compiler cannot sink a buffer's loads to its use
-    Buf b0, b1;
+    // four load buffers, the loads running two steps ahead into registers that the steps in
+    // between do not read; the step count is a multiple of 4 (KL is a multiple of 16), so every
+    // compute is unconditional and the compiler cannot sink a buffer's loads to its use
+    Buf b0, b1, b2, b3;
     load(b0, 0);
-    for (int s = 0; s < nsteps; s += 2) {
-        load(b1, s + 1);
+    load(b1, 1);
+    for (int s = 0; s < nsteps; s += 4) {
+        load(b2, s + 2);
+        load(b3, s + 3);
+        __builtin_amdgcn_sched_barrier(0);
         compute(b0, s);
-        load(b0, s + 2);
         compute(b1, s + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        load(b0, s + 4);
+        load(b1, s + 5);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(b2, s + 2);
+        compute(b3, s + 3);
+        __builtin_amdgcn_sched_barrier(0);
     }
     // this lane holds (re, im) of complex column qout(k, h) in registers (2h, 2h + 1), for
     // m = m0 + 16t + col
@@ -692,8 +701,9 @@ static void y_shape(YArgs &a) {
     a.has_mid = (a.NY2 % 2 == 0) ? 1 : 0;
     a.rows_m = a.NLH - a.has_mid;
     a.njp = (a.nj + 7) / 8 * 8;
-    // forward table rows: a multiple of 8, so that the forward kernel's k-steps (4 rows each) pair up
-    a.KL = (a.rows_m + 7) / 8 * 8;
+    // forward table rows: a multiple of 16, so that the forward kernel's k-steps (4 rows each) come
+    // in groups of four
+    a.KL = (a.rows_m + 15) / 16 * 16;
 }
 
 // largest tile count per part, at most `cap`, whose table fits the LDS budget
