@@ -258,6 +258,22 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
         tau_solve<NE, 3>(forcing, vc, adv, WL, WD, WR, lane, ny, wn_v, a.g_v, av, bv, zero, vlo, vup);
     }
 
+    // ---- per-column data of the step's corrections (influence entries, sigma denominators), read
+    // before the wall divergence so that its reductions cover the latency
+    const bool compact = a.compact && mode == MODE_STEP;
+    zd idc[4] = {}, sgc[2] = {};
+    if (compact && a.correct) {
+        const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
+        idc[0] = I0[0];
+        idc[1] = I0[1];
+        idc[2] = I1[0];
+        idc[3] = I1[1];
+        if (a.sigma) {
+            sgc[0] = a.SIG0[(size_t)q * 2];
+            sgc[1] = a.SIG0[(size_t)q * 2 + 1];
+        }
+    }
+
     // ---- wall divergence (eqsolve_pv.cpp:397-406)
     zd div0 = {0.0, 0.0}, div1 = {0.0, 0.0};
     if (a.correct || mode == MODE_INFLUENCE) {
@@ -288,25 +304,31 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
 
     // ---- influence-matrix and sigma corrections (eqsolve_pv.cpp:407-445)
     zd c0 = {0.0, 0.0}, c1 = {0.0, 0.0}, sig[2] = {{0.0, 0.0}, {0.0, 0.0}};
-    const bool compact = a.compact && mode == MODE_STEP;
     if (a.correct) {
         const zd *I0 = a.ID[0] + (size_t)q * 2, *I1 = a.ID[1] + (size_t)q * 2;
-        c0 = cmul(div1, I1[0]) - cmul(div0, I1[1]);
-        c1 = cmul(div0, I0[1]) - cmul(div1, I0[0]);
+        const zd i00 = compact ? idc[0] : I0[0], i01 = compact ? idc[1] : I0[1];
+        const zd i10 = compact ? idc[2] : I1[0], i11 = compact ? idc[3] : I1[1];
+        c0 = cmul(div1, i10) - cmul(div0, i11);
+        c1 = cmul(div0, i01) - cmul(div1, i00);
         if (a.sigma) {
             zd ptop, vt0, vt1;
+            zd s0, s1;
             if (compact) {
                 ptop = sp::col_at<NE>(pc[0], ny) + (a.PIr[0][off + ny] * c0 + a.PIr[1][off + ny] * c1);
                 vt0 = sp::col_at<NE>(vc[1], ny) + (a.VIv[0][off + ny] * c0 + a.VIv[1][off + ny] * c1);
                 vt1 = sp::col_at<NE>(vc[1], ny - 1) + (a.VIv[0][off + ny - 1] * c0 + a.VIv[1][off + ny - 1] * c1);
+                s0 = sgc[0];
+                s1 = sgc[1];
             } else {
                 ptop = sp::col_at<NE>(pc[0], ny) + (cmul(c0, a.PI[0][off + ny]) + cmul(c1, a.PI[1][off + ny]));
                 vt0 = sp::col_at<NE>(vc[1], ny) + (cmul(c0, a.VI[0][1][off + ny]) + cmul(c1, a.VI[1][1][off + ny]));
                 vt1 = sp::col_at<NE>(vc[1], ny - 1) +
                       (cmul(c0, a.VI[0][1][off + ny - 1]) + cmul(c1, a.VI[1][1][off + ny - 1]));
+                s0 = a.SIG0[(size_t)q * 2 + 0];
+                s1 = a.SIG0[(size_t)q * 2 + 1];
             }
-            sig[0] = cdiv((-wn_v) * vt0 - a.Re * tail0, a.SIG0[(size_t)q * 2 + 0]);
-            sig[1] = cdiv((-wn_v) * vt1 + a.Re * ((2.0 * ny) * ptop - tail1), a.SIG0[(size_t)q * 2 + 1]);
+            sig[0] = cdiv((-wn_v) * vt0 - a.Re * tail0, s0);
+            sig[1] = cdiv((-wn_v) * vt1 + a.Re * ((2.0 * ny) * ptop - tail1), s1);
         }
     }
     // the B response enters with sigma_(1 - p) for P, u, w and sigma_p for v (q = (j+1) mod 2)
