@@ -53,8 +53,8 @@ def algorithmic_bytes(cfg, esz):
 # (FP64), each scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0); profiles/step_retau180_hbm_traffic.md holds the runs.
 MEASURED_TRAFFIC = {
-    "x_product": 356.2e6,
-    "solve_step": 373.8e6,
+    "x_product": 356.3e6,
+    "solve_step": 374.4e6,
 }
 
 
