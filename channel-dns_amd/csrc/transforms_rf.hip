@@ -406,7 +406,9 @@ template <typename R, int N1, int N2>
 __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
     using Gm = Geo<N1, N2>;
     constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
-    __shared__ cpx<R> tw[N];
+    // the root table is read from global memory (cache-resident), not staged in LDS: the transpose
+    // buffers alone set the blocks per CU
+    const cpx<R> *tw = reinterpret_cast<const cpx<R> *>(a.tw);
     __shared__ cpx<R> buf[NG][LB];
     const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
     // neighbouring kx groups of one (field, l) read the same cache lines: keep them on one XCD
@@ -421,8 +423,6 @@ __global__ void __launch_bounds__(64) k_zfwd_rf(ZArgs a) {
     cpx<R> x[MX];
     sfor<0, MX>([&](auto j) { x[j] = cpx<R>{R(0), R(0)}; });
     if (valid && t < N1) sfor<0, N2>([&](auto n2) { x[n2] = Z[t + N1 * decltype(n2)::value]; });
-    load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
-    __syncthreads();
     two_pass<R, N1, N2, -1>(x, t, tw, buf[g], lsync);
     if (valid && t < N2) {
         const R inv = R(1.0 / N);
