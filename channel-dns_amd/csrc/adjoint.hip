@@ -1,7 +1,7 @@
 // Discrete adjoint of the channel step on gfx950 (FP64).
 //
 // The reference has no adjoint.  These kernels transpose, operation by operation, the step the
-// forward kernels compute (src/sim/simulation.cpp:204-233 as restated by the forward path), so
+// forward kernels compute (src/sim/simulation.cpp:142-171 as restated by the forward path), so
 // that <J dX, L> = <dX, J^T L> holds to round-off with real inner products on (re, im) pairs:
 //   k_adj_solve    flow correction^T, zero mode^T, solve_modes^T (influence and sigma rank-2
 //                  updates, four transposed tau solves, eqsolve_pv.cpp:16-60, 311-450), the

@@ -161,6 +161,11 @@ struct cdns_ctx {
     zd *a_lNs[3] = {}, *a_lw[3] = {}, *a_lUs[3] = {}, *a_lWs[3] = {};
     void *ZL = nullptr, *ZNa = nullptr;
     double *Jpart = nullptr;
+    zd *body_save[3] = {}; // the caller's body force while cdns_adjoint_run uses its own
+    // horizon storage of cdns_adjoint_run (checkpoints, one segment's synthesised fields), kept
+    // between calls and grown when a call needs more
+    void *hz_ck = nullptr, *hz_seg = nullptr;
+    size_t hz_ck_bytes = 0, hz_seg_bytes = 0;
 
     // coarse grid NX x (NY+1) x NZ (Transforms::channel_xyzfft): FP64 tables and plans, built on
     // first use; the transforms run in the product-grid workspaces, which are larger
@@ -691,12 +696,17 @@ void step_coefficients(int it, double dt, double Re, double co[5]) {
     co[4] = (a0 + a1) * Re;
 }
 
-// One iteration of Simulation::run (simulation.cpp:235-247) with enqueue_serial_step (204-233).
+// One iteration of Simulation::run (simulation.cpp:173-191) with enqueue_serial_step (142-171).
 void step(cdns_ctx *c, int it) {
     double co[5];
     step_coefficients(it, c->cfg.DT, c->cfg.Re, co);
-    if (it < 3) setup(c, co[4]);
-    if (!c->setup_done) throw std::logic_error("solver setup has not run (the first steps must be 1 and 2)");
+    if (it < 3) {
+        setup(c, co[4]);
+    } else {
+        if (!c->setup_done) throw std::logic_error("solver setup has not run (the first steps must be 1 and 2)");
+        // the tables may have been set up for another g since (an adjoint step of iteration 1)
+        if (c->g_v != co[4]) setup(c, co[4]);
+    }
     c->time += c->cfg.DT;
     nonlinear_planes(c, c->u, c->nl);
     SolveArgs a = solve_args(c, MODE_STEP);
@@ -758,6 +768,7 @@ void ensure_adjoint(cdns_ctx *c) {
     c->ZL = c->alloc<unsigned char>(3 * g.ZS * c->esz);
     c->ZNa = c->alloc<unsigned char>(6 * g.ZS * c->esz);
     c->Jpart = c->alloc<double>(g.NQ);
+    for (int k = 0; k < 3; ++k) c->body_save[k] = c->alloc<zd>(g.S);
     c->adj_ready = true;
 }
 
@@ -968,14 +979,135 @@ int fail(cdns_ctx *c, const std::exception &e) {
     return -1;
 }
 
+// Storage of one adjoint horizon: a checkpoint (u, u_prev, N_prev: 9 planes, and B_prev) every
+// k_ckpt steps, and one segment's synthesised (u, curl u) in the Z layout per step.
+struct HorizonSize {
+    int kc = 0, nck = 0;
+    size_t ckn = 0;       // complex entries per checkpoint
+    size_t ck_bytes = 0;  // all checkpoints
+    size_t seg_bytes = 0; // one segment
+};
+
+HorizonSize horizon_size(const cdns_ctx *c, int nsteps, int k_ckpt) {
+    if (nsteps < 1 || k_ckpt < 1) throw std::invalid_argument("nsteps and the checkpoint interval must be >= 1");
+    const Geometry &g = c->g;
+    HorizonSize h;
+    h.kc = std::min(k_ckpt, nsteps);
+    h.nck = (nsteps + h.kc - 1) / h.kc;
+    h.ckn = (size_t)9 * g.S + (size_t)g.NQ * 2;
+    h.ck_bytes = (size_t)h.nck * h.ckn * sizeof(zd);
+    h.seg_bytes = (size_t)h.kc * 6 * g.ZS * c->esz;
+    return h;
+}
+
+// Gradient of objective(u(T)) over iterations 1..nsteps from u0 with the body force `force` (or
+// none), with the horizon storage at ck and seg (device memory of horizon_size's sizes).  The
+// context's state is overwritten; its time and body force are what they were before.
+void adjoint_horizon(cdns_ctx *c, const double *u0, const double *force, int nsteps, int k_ckpt, int objective_id,
+                     double *grad_u0, double *grad_force, double *J, int where, void *ck, void *seg) {
+    const Geometry &g = c->g;
+    const HorizonSize hs = horizon_size(c, nsteps, k_ckpt);
+    ensure_adjoint(c);
+    const double time0 = c->time;
+    const bool had_body = c->body_on;
+    void *const zkeep = c->Zbuf;
+    const size_t bplane = g.S * sizeof(zd);
+    auto restore_context = [&]() {
+        c->Zbuf = zkeep;
+        c->time = time0;
+        if (force && had_body)
+            for (int k = 0; k < 3; ++k)
+                CDNS_HIP(hipMemcpyAsync(c->body[k], c->body_save[k], bplane, hipMemcpyDeviceToDevice, c->st));
+        c->body_on = had_body;
+    };
+    try {
+        if (force) {
+            if (had_body)
+                for (int k = 0; k < 3; ++k)
+                    CDNS_HIP(hipMemcpyAsync(c->body_save[k], c->body[k], bplane, hipMemcpyDeviceToDevice, c->st));
+            if (cdns_set_body_force(c, force, where) != 0) throw std::runtime_error(c->err);
+        } else {
+            c->body_on = false;
+        }
+        // initial state: u0, no history (steps 1 and 2 weight it by zero)
+        ref_to_planes(c, u0, 3, 3, g.nj, c->u, where, g.JS);
+        zero_planes(c, c->up, 3, g.S);
+        zero_planes(c, c->nlp, 3, g.S);
+        CDNS_HIP(hipMemsetAsync(c->bc_prev, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
+        const int kc = hs.kc, nck = hs.nck;
+        const size_t ckn = hs.ckn;
+        auto save = [&](int k) {
+            zd *base = reinterpret_cast<zd *>(ck) + (size_t)k * ckn;
+            for (int i = 0; i < 3; ++i) {
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)i * g.S, c->u[i], bplane, hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)(3 + i) * g.S, c->up[i], bplane, hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(base + (size_t)(6 + i) * g.S, c->nlp[i], bplane, hipMemcpyDeviceToDevice, c->st));
+            }
+            CDNS_HIP(hipMemcpyAsync(base + (size_t)9 * g.S, c->bc_prev, (size_t)g.NQ * 2 * sizeof(zd),
+                                    hipMemcpyDeviceToDevice, c->st));
+        };
+        auto restore = [&](int k) {
+            const zd *base = reinterpret_cast<const zd *>(ck) + (size_t)k * ckn;
+            for (int i = 0; i < 3; ++i) {
+                CDNS_HIP(hipMemcpyAsync(c->u[i], base + (size_t)i * g.S, bplane, hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(c->up[i], base + (size_t)(3 + i) * g.S, bplane, hipMemcpyDeviceToDevice, c->st));
+                CDNS_HIP(hipMemcpyAsync(c->nlp[i], base + (size_t)(6 + i) * g.S, bplane, hipMemcpyDeviceToDevice, c->st));
+            }
+            CDNS_HIP(hipMemcpyAsync(c->bc_prev, base + (size_t)9 * g.S, (size_t)g.NQ * 2 * sizeof(zd),
+                                    hipMemcpyDeviceToDevice, c->st));
+        };
+        for (int it = 1; it <= nsteps; ++it) {
+            if ((it - 1) % kc == 0) save((it - 1) / kc);
+            step(c, it);
+        }
+        // terminal objective and its gradient start the adjoint
+        const double Jv = objective(c, objective_id, c->u, c->a_lu);
+        if (J) *J = Jv;
+        zero_planes(c, c->a_lup, 3, g.S);
+        zero_planes(c, c->a_lnp, 3, g.S);
+        zero_planes(c, c->a_lf, 3, g.S);
+        CDNS_HIP(hipMemsetAsync(c->a_lbp, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
+        const size_t zstep = (size_t)6 * g.ZS * c->esz;
+        for (int k = nck - 1; k >= 0; --k) {
+            const int it0 = k * kc + 1, it1 = std::min((k + 1) * kc, nsteps);
+            restore(k);
+            // recompute the segment, keeping each step's synthesised (u, curl u)
+            for (int it = it0; it <= it1; ++it) {
+                c->Zbuf = (char *)seg + (size_t)(it - it0) * zstep;
+                step(c, it);
+            }
+            c->Zbuf = zkeep;
+            for (int it = it1; it >= it0; --it)
+                adjoint_step(c, it, (const char *)seg + (size_t)(it - it0) * zstep, force != nullptr);
+        }
+        restore_context();
+        // leave the tables of the steady iterations in place for later forward steps
+        if (nsteps >= 2) ensure_tables(c, g_for_iteration(c, 2));
+        const zd *o1[3] = {c->a_lu[0], c->a_lu[1], c->a_lu[2]};
+        planes_to_ref(c, o1, 3, g.nj, grad_u0, 3, where, g.JS);
+        if (grad_force) {
+            const zd *o2[3] = {c->a_lf[0], c->a_lf[1], c->a_lf[2]};
+            planes_to_ref(c, o2, 3, g.nj, grad_force, 3, where, g.JS);
+        }
+        CDNS_HIP(hipStreamSynchronize(c->st));
+    } catch (...) {
+        (void)hipStreamSynchronize(c->st);
+        c->Zbuf = zkeep;
+        c->time = time0;
+        c->body_on = had_body && !force; // a body force half-replaced is not restored
+        throw;
+    }
+}
+
 } // namespace
 
 // ======================================================================= ABI
+
 extern "C" {
 
 int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
+    if (out) *out = nullptr;
     if (!cfg || !out) return -1;
-    *out = nullptr;
     auto c = std::make_unique<cdns_ctx>();
     try {
         c->cfg = *cfg;
@@ -1162,6 +1294,8 @@ void cdns_destroy(cdns_ctx *c) {
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->hz_ck) (void)hipFree(c->hz_ck);
+    if (c->hz_seg) (void)hipFree(c->hz_seg);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -1694,91 +1828,32 @@ int cdns_adjoint_step(cdns_ctx *c, int it, const double *velocity, double *adj_v
 
 int cdns_adjoint_run(cdns_ctx *c, const double *u0, const double *force, int nsteps, int k_ckpt, int objective_id,
                      double *grad_u0, double *grad_force, double *J, int where) {
-    void *ck = nullptr, *seg = nullptr;
-    void *zkeep = c ? c->Zbuf : nullptr;
     try {
-        const Geometry &g = c->g;
-        if (nsteps < 1 || k_ckpt < 1) throw std::invalid_argument("nsteps and the checkpoint interval must be >= 1");
+        if (!c) throw std::invalid_argument("null context");
         ensure_adjoint(c);
-        if (force) {
-            if (cdns_set_body_force(c, force, where) != 0) throw std::runtime_error(c->err);
-        } else {
-            c->body_on = false;
+        const HorizonSize hs = horizon_size(c, nsteps, k_ckpt);
+        // the horizon storage stays with the context and grows when a call needs more
+        if (hs.ck_bytes > c->hz_ck_bytes) {
+            CDNS_HIP(hipStreamSynchronize(c->st));
+            if (c->hz_ck) CDNS_HIP(hipFree(c->hz_ck));
+            c->hz_ck = nullptr;
+            c->hz_ck_bytes = 0;
+            CDNS_HIP(hipMalloc(&c->hz_ck, hs.ck_bytes));
+            c->hz_ck_bytes = hs.ck_bytes;
         }
-        // initial state: u0, no history (steps 1 and 2 weight it by zero)
-        ref_to_planes(c, u0, 3, 3, g.nj, c->u, where, g.JS);
-        zero_planes(c, c->up, 3, g.S);
-        zero_planes(c, c->nlp, 3, g.S);
-        CDNS_HIP(hipMemsetAsync(c->bc_prev, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
-        const int kc = std::min(k_ckpt, nsteps);
-        const int nck = (nsteps + kc - 1) / kc;
-        // checkpoint: u, u_prev, N_prev (9 planes) and B_prev
-        const size_t ckn = (size_t)9 * g.S + (size_t)g.NQ * 2;
-        CDNS_HIP(hipMalloc(&ck, (size_t)nck * ckn * sizeof(zd)));
-        CDNS_HIP(hipMalloc(&seg, (size_t)kc * 6 * g.ZS * c->esz));
-        auto save = [&](int k) {
-            zd *base = reinterpret_cast<zd *>(ck) + (size_t)k * ckn;
-            for (int i = 0; i < 3; ++i) {
-                CDNS_HIP(hipMemcpyAsync(base + (size_t)i * g.S, c->u[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-                CDNS_HIP(hipMemcpyAsync(base + (size_t)(3 + i) * g.S, c->up[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-                CDNS_HIP(hipMemcpyAsync(base + (size_t)(6 + i) * g.S, c->nlp[i], g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-            }
-            CDNS_HIP(hipMemcpyAsync(base + (size_t)9 * g.S, c->bc_prev, (size_t)g.NQ * 2 * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-        };
-        auto restore = [&](int k) {
-            const zd *base = reinterpret_cast<const zd *>(ck) + (size_t)k * ckn;
-            for (int i = 0; i < 3; ++i) {
-                CDNS_HIP(hipMemcpyAsync(c->u[i], base + (size_t)i * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-                CDNS_HIP(hipMemcpyAsync(c->up[i], base + (size_t)(3 + i) * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-                CDNS_HIP(hipMemcpyAsync(c->nlp[i], base + (size_t)(6 + i) * g.S, g.S * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-            }
-            CDNS_HIP(hipMemcpyAsync(c->bc_prev, base + (size_t)9 * g.S, (size_t)g.NQ * 2 * sizeof(zd), hipMemcpyDeviceToDevice, c->st));
-        };
-        auto fwd = [&](int it) {
-            if (it >= 3) ensure_tables(c, g_for_iteration(c, it));
-            step(c, it);
-        };
-        for (int it = 1; it <= nsteps; ++it) {
-            if ((it - 1) % kc == 0) save((it - 1) / kc);
-            fwd(it);
+        if (hs.seg_bytes > c->hz_seg_bytes) {
+            CDNS_HIP(hipStreamSynchronize(c->st));
+            if (c->hz_seg) CDNS_HIP(hipFree(c->hz_seg));
+            c->hz_seg = nullptr;
+            c->hz_seg_bytes = 0;
+            CDNS_HIP(hipMalloc(&c->hz_seg, hs.seg_bytes));
+            c->hz_seg_bytes = hs.seg_bytes;
         }
-        // terminal objective and its gradient start the adjoint
-        const double Jv = objective(c, objective_id, c->u, c->a_lu);
-        if (J) *J = Jv;
-        zero_planes(c, c->a_lup, 3, g.S);
-        zero_planes(c, c->a_lnp, 3, g.S);
-        zero_planes(c, c->a_lf, 3, g.S);
-        CDNS_HIP(hipMemsetAsync(c->a_lbp, 0, (size_t)g.NQ * 2 * sizeof(zd), c->st));
-        for (int k = nck - 1; k >= 0; --k) {
-            const int it0 = k * kc + 1, it1 = std::min((k + 1) * kc, nsteps);
-            restore(k);
-            // recompute the segment, keeping each step's synthesised (u, curl u)
-            for (int it = it0; it <= it1; ++it) {
-                c->Zbuf = (char *)seg + (size_t)(it - it0) * 6 * g.ZS * c->esz;
-                fwd(it);
-            }
-            c->Zbuf = zkeep;
-            for (int it = it1; it >= it0; --it)
-                adjoint_step(c, it, (const char *)seg + (size_t)(it - it0) * 6 * g.ZS * c->esz, force != nullptr);
-        }
-        c->Zbuf = zkeep;
-        // leave the tables of the steady iterations in place for later forward steps
-        if (nsteps >= 2) ensure_tables(c, g_for_iteration(c, 2));
-        const zd *o1[3] = {c->a_lu[0], c->a_lu[1], c->a_lu[2]};
-        planes_to_ref(c, o1, 3, g.nj, grad_u0, 3, where, g.JS);
-        if (grad_force) {
-            const zd *o2[3] = {c->a_lf[0], c->a_lf[1], c->a_lf[2]};
-            planes_to_ref(c, o2, 3, g.nj, grad_force, 3, where, g.JS);
-        }
-        CDNS_HIP(hipStreamSynchronize(c->st));
-        CDNS_HIP(hipFree(ck));
-        CDNS_HIP(hipFree(seg));
+        adjoint_horizon(c, u0, force, nsteps, k_ckpt, objective_id, grad_u0, grad_force, J, where, c->hz_ck,
+                        c->hz_seg);
         return 0;
     } catch (const std::exception &e) {
-        if (c) c->Zbuf = zkeep;
         if (c && c->st) (void)hipStreamSynchronize(c->st);
-        if (ck) (void)hipFree(ck);
-        if (seg) (void)hipFree(seg);
         return fail(c, e);
     }
 }

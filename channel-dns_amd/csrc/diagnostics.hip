@@ -6,7 +6,7 @@
 
 namespace cdns {
 
-// vector_ops::divergence (vector_ops.cpp:128-143): i kx u + D v + i kz w; one wavefront per column.
+// vector_ops::divergence (vector_ops.cpp:90-105): i kx u + D v + i kz w; one wavefront per column.
 template <int NE>
 __global__ void __launch_bounds__(256) k_divergence(DiagArgs a) {
     const int lane = lane_id();

@@ -107,7 +107,7 @@ struct CurlArgs {
     int ny, JS, NQ, nzk;
 };
 void launch_curl(const CurlArgs &a, int E, hipStream_t st);
-// pressure wall condition = curl part + sum_j (+-1)^j N_y,j (timestepper.cpp:55-72)
+// pressure wall condition = curl part + sum_j (+-1)^j N_y,j (timestepper.cpp:26-43)
 void launch_wall_sum(const zd *nly, const zd *bcw, zd *pbc, int ny, int JS, int NQ, int E, hipStream_t st);
 
 // ------------------------------------------------------------- solver ----

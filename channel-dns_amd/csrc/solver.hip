@@ -121,9 +121,9 @@ __device__ void tau_solve(const zd (&r)[NR][NE], zd (&out)[NR][NE], const double
 }
 
 // Fused coupled solve of one (kx,kz) pair: EqSolvePV::solve_modes (eqsolve_pv.cpp:311-450),
-// with the step's RHS assembly (simulation.cpp:204-221), the nonlinear pressure wall condition
-// (timestepper.cpp:55-72), correct_zero_mode (timestepper.cpp:89-97) and correct_flow
-// (simulation.cpp:173-202) fused in MODE_STEP; the setup problems of eqsolve_pv.cpp:266-303
+// with the step's RHS assembly (simulation.cpp:144-160), the nonlinear pressure wall condition
+// (timestepper.cpp:26-43), correct_zero_mode (timestepper.cpp:60-68) and correct_flow
+// (simulation.cpp:111-140) fused in MODE_STEP; the setup problems of eqsolve_pv.cpp:266-303
 // and 452-477 in MODE_INFLUENCE / MODE_BPROBLEM.
 template <int NE, int MODE>
 __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {

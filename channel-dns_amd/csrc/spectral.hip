@@ -5,8 +5,8 @@
 
 namespace cdns {
 
-// vector_ops::enqueue_curl (src/solver/vector_ops.cpp:150-189) plus the curl terms of
-// TimeStepper::enqueue_pressure_boundary (src/solver/timestepper.cpp:55-72):
+// vector_ops::enqueue_curl (src/solver/vector_ops.cpp:112-151) plus the curl terms of
+// TimeStepper::enqueue_pressure_boundary (src/solver/timestepper.cpp:26-43):
 //   wx = D w - i kz v,  wy = i kz u - i kx w,  wz = i kx v - D u,
 //   bcw(wall 0) = sum_j t_j,  bcw(wall 1) = sum_j (-1)^(j+1) t_j,  t_j = (i kx/Re) wz_j - (i kz/Re) wx_j.
 // One wavefront per (kx,kz) column.

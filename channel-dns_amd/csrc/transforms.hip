@@ -2,7 +2,7 @@
 //
 // The reference runs y REDFT00 -> identity transpose -> z c2c -> x c2r per component
 // (src/fft/transforms.cpp:687-740) and the inverse chain (641-685), with the point-wise
-// cross product as a separate pass (src/solver/vector_ops.cpp:288-310).  Here:
+// cross product as a separate pass (src/solver/vector_ops.cpp:250-272).  Here:
 //   y stage : dense Chebyshev matrix apply split by parity (T_j(-y) = (-1)^j T_j(y)),
 //             LDS-tiled, one complex column per lane;
 //   z stage : mixed-radix Stockham FFT over kz in LDS, kx lines batched per workgroup;

@@ -44,7 +44,7 @@ typedef struct cdns_config {
 typedef struct cdns_ctx cdns_ctx;
 
 /* Replaces the construction of Transforms + EqSolvePV + TimeStepper + Simulation
- * buffers (src/main.cpp:192-221; src/sim/simulation.cpp:109-129). */
+ * buffers (src/main.cpp:192-221; src/sim/simulation.cpp:47-67). */
 int cdns_create(const cdns_config *cfg, cdns_ctx **out);
 void cdns_destroy(cdns_ctx *ctx);
 const char *cdns_last_error(const cdns_ctx *ctx);
@@ -102,12 +102,14 @@ int cdns_get_table(cdns_ctx *ctx, int which, double *out);
  * The multistep history (1, 3, 6) is what a restart file does not hold. */
 int cdns_set_state(cdns_ctx *ctx, int field, const double *data, int where);
 int cdns_get_state(cdns_ctx *ctx, int field, double *data, int where);
-/* One iteration `it` (>= 1) of Simulation::run's serial body: step_coefficients, setup on
- * it < 3, enqueue_serial_step and correct_flow (src/sim/simulation.cpp:204-247). */
+/* One iteration `it` (>= 1) of Simulation::run's serial body (src/sim/simulation.cpp:173-191):
+ * step_coefficients (:36-45), setup on it < 3 (:176-177), enqueue_serial_step (:142-171) with
+ * correct_flow (:111-140).  From it = 3 on, the solver tables are those of the steady g; they are
+ * set up again if another call (cdns_setup, cdns_adjoint_step of iteration 1) changed them. */
 int cdns_step(cdns_ctx *ctx, int it);
 /* Iterations it_begin .. it_begin+nsteps-1. */
 int cdns_run(cdns_ctx *ctx, int it_begin, int nsteps);
-/* dpdx_, dpdz_ after the last step (simulation.cpp:173-202) and the simulated time. */
+/* dpdx_, dpdz_ after the last step (simulation.cpp:111-140) and the simulated time. */
 int cdns_flow_gradients(cdns_ctx *ctx, double out[2]);
 double cdns_time(const cdns_ctx *ctx);
 
@@ -141,7 +143,11 @@ int cdns_adjoint_step(cdns_ctx *ctx, int it, const double *velocity, double *adj
 /* Gradient of objective(u(T)) after iterations 1..nsteps from u0 (zero history) with body force
  * `force` (or none): forward with a checkpoint every k_ckpt steps, then per segment a forward
  * recomputation that keeps each step's synthesised (u, curl u), and the adjoint steps in reverse.
- * Writes J, grad_u0 and (if non-NULL) grad_force.  The context's state is overwritten. */
+ * Writes J, grad_u0 and (if non-NULL) grad_force.  The context's state (velocity and history) is
+ * overwritten; its time and its body force (cdns_set_body_force) are restored.  The checkpoints
+ * and one segment's fields stay allocated with the context between calls (grown when a call needs
+ * more): (ceil(nsteps / k_ckpt) (9 S + 2 NQ) + k_ckpt * 6 Z) * 16 bytes, S, Z and NQ the spectral,
+ * z-physical and column counts. */
 int cdns_adjoint_run(cdns_ctx *ctx, const double *u0, const double *force, int nsteps, int k_ckpt,
                      int objective, double *grad_u0, double *grad_force, double *J, int where);
 

@@ -1,7 +1,7 @@
 /* Tangent-linear and discrete adjoint of the channel-dns time step -- TEST INFRASTRUCTURE.
  *
  * The reference has no adjoint; this file differentiates the oracle's own step
- * (cdns_oracle.c, which restates src/sim/simulation.cpp:204-233) operation by operation and
+ * (cdns_oracle.c, which restates src/sim/simulation.cpp:142-171) operation by operation and
  * transposes each linear piece exactly, so that <TL dx, l> = <dx, ADJ l> holds to round-off.
  * Inner products are real: complex arrays are vectors of (re, im) pairs, physical fields
  * plain real vectors.  Transposes of the pieces:

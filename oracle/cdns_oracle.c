@@ -557,7 +557,7 @@ void orc_derivative(orc_ctx *c, int axis, double *outd, const double *ind) {
     cplx *out = (cplx *)outd;
     const cplx *in = (const cplx *)ind;
     const int NYS = c->cfg.NYS;
-    /* vector_ops.cpp:78-109 */
+    /* vector_ops.cpp:40-72 */
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi)
         for (int kx = 0; kx < c->nkx; ++kx) {
@@ -575,7 +575,7 @@ void orc_gradient(orc_ctx *c, double *outd, const double *ind) {
     const cplx *in = (const cplx *)ind;
     const int NYS = c->cfg.NYS;
     const size_t col = (size_t)c->nkx;
-    /* vector_ops.cpp:111-126 */
+    /* vector_ops.cpp:73-88 */
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi)
         for (int kx = 0; kx < c->nkx; ++kx) {
@@ -592,7 +592,7 @@ void orc_divergence(orc_ctx *c, double *outd, const double *ind) {
     const cplx *in = (const cplx *)ind;
     const int NYS = c->cfg.NYS;
     const size_t col = (size_t)c->nkx;
-    /* vector_ops.cpp:128-143 */
+    /* vector_ops.cpp:90-105 */
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi)
         for (int kx = 0; kx < c->nkx; ++kx) {
@@ -608,7 +608,7 @@ void orc_curl(orc_ctx *c, double *outd, const double *ind) {
     cplx *out = (cplx *)outd;
     const cplx *in = (const cplx *)ind;
     const int NYS = c->cfg.NYS;
-    /* enqueue_curl (vector_ops.cpp:150-189) */
+    /* enqueue_curl (vector_ops.cpp:112-151) */
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi) {
         const double kz = c->WAVZ[izi];
@@ -644,7 +644,7 @@ void orc_curl(orc_ctx *c, double *outd, const double *ind) {
 void orc_divergence_boundary(orc_ctx *c, double *outd, const double *ind) {
     cplx *out = (cplx *)outd;
     const cplx *in = (const cplx *)ind;
-    /* vector_ops.cpp:197-214 */
+    /* vector_ops.cpp:159-176 */
 #pragma omp parallel for schedule(static)
     for (int izi = 0; izi < c->nzk; ++izi)
         for (int x = 0; x < c->nkx; ++x) {
@@ -664,7 +664,7 @@ void orc_divergence_boundary(orc_ctx *c, double *outd, const double *ind) {
 double orc_flow_rate(const orc_ctx *c, const double *fieldd, int stride, int comp) {
     const cplx *f = (const cplx *)fieldd;
     double q = 0.0;
-    /* vector_ops.cpp:224-229, simulation.cpp:166-171 */
+    /* vector_ops.cpp:186-191, simulation.cpp:104-109 */
     for (int iy = 0; iy <= c->cfg.NYS; iy += 2)
         q -= creal(f[sidx(c, c->zoff, iy, 0) * stride + comp]) / (double)(iy * iy - 1);
     return q;
@@ -675,7 +675,7 @@ static size_t fine_phys_size(const orc_ctx *c) {
 }
 
 void orc_cross_product(orc_ctx *c, double *outd, const double *ad, const double *bd) {
-    /* enqueue_cross_product (vector_ops.cpp:288-310) */
+    /* enqueue_cross_product (vector_ops.cpp:250-272) */
     const size_t n = fine_phys_size(c);
     double *ap[3], *bp[3], *pr[3];
     for (int k = 0; k < 3; ++k) {
@@ -700,7 +700,7 @@ void orc_cross_product(orc_ctx *c, double *outd, const double *ad, const double 
 }
 
 void orc_nonlinear_term(orc_ctx *c, double *nld, const double *veld, double *pbcd) {
-    /* TimeStepper::enqueue_nonlinear_term (timestepper.cpp:48-72) */
+    /* TimeStepper::enqueue_nonlinear_term (timestepper.cpp:19-43) */
     cplx *nl = (cplx *)nld, *bc = (cplx *)pbcd;
     orc_curl(c, (double *)c->curl_ws, veld);
     orc_cross_product(c, nld, veld, (const double *)c->curl_ws);
@@ -1073,7 +1073,7 @@ void orc_get_table(const orc_ctx *c, int which, double *out) {
 }
 
 /* ----------------------------------------------------------- simulation ---- */
-/* step_coefficients (simulation.cpp:98-107) */
+/* step_coefficients (simulation.cpp:36-45) */
 void orc_step_coefficients(int it, double dt, double Re, double out[5]) {
     const int eff = it < 3 ? it : 2;
     const double a0 = (double)eff / dt, a1 = (double)(1 - eff) / (2.0 * dt);
@@ -1091,14 +1091,14 @@ static size_t field_size(const orc_ctx *c, int f) {
 void orc_set_state(orc_ctx *c, int f, const double *d) { memcpy(c->st[f], d, field_size(c, f) * sizeof(cplx)); }
 void orc_get_state(const orc_ctx *c, int f, double *d) { memcpy(d, c->st[f], field_size(c, f) * sizeof(cplx)); }
 
-/* correct_zero_mode (timestepper.cpp:89-97) */
+/* correct_zero_mode (timestepper.cpp:60-68) */
 static void correct_zero_mode(orc_ctx *c, cplx *vel, const cplx *vbc) {
     const cplx lower = vbc[bidx(c, c->zoff, 0, 0) * 3 + 1], upper = vbc[bidx(c, c->zoff, 1, 0) * 3 + 1];
     vel[sidx(c, c->zoff, 0, 0) * 3 + 1] = cmulr(lower + upper, 0.5);
     for (int iy = 1; iy <= c->cfg.NYS; ++iy) vel[sidx(c, c->zoff, iy, 0) * 3 + 1] = 0;
 }
 
-/* correct_flow (simulation.cpp:173-202) */
+/* correct_flow (simulation.cpp:111-140) */
 static void correct_flow(orc_ctx *c) {
     const orc_config *m = &c->cfg;
     cplx *vel = c->st[F_VEL];
@@ -1124,7 +1124,7 @@ void orc_step(orc_ctx *c, int it) {
     orc_step_coefficients(it, m->DT, m->Re, co);
     if (it < 3) orc_setup(c, co[4]);
     c->time += m->DT;
-    /* enqueue_serial_step (simulation.cpp:204-233) */
+    /* enqueue_serial_step (simulation.cpp:142-171) */
     orc_nonlinear_term(c, (double *)c->st[F_NL], (const double *)c->st[F_VEL], (double *)c->st[F_PBC]);
     for (size_t s = 0; s < c->BND; ++s)
         c->st[F_PBC_PREV][s] = cmulr(c->st[F_PBC][s], co[2]) + cmulr(c->st[F_PBC_PREV][s], co[3]);

@@ -62,7 +62,7 @@ void orc_divergence_boundary(orc_ctx *c, double *out, const double *in);
 double orc_flow_rate(const orc_ctx *c, const double *scalar_or_vector, int stride, int comp);
 void orc_cross_product(orc_ctx *c, double *out, const double *a, const double *b);
 
-/* TimeStepper::nonlinear_term (timestepper.cpp:40-72): writes nl and the pressure BC. */
+/* TimeStepper::nonlinear_term (timestepper.cpp:11-43): writes nl and the pressure BC. */
 void orc_nonlinear_term(orc_ctx *c, double *nl, const double *velocity, double *pressure_bc);
 
 /* EqSolvePV (eqsolve_pv.cpp). */
@@ -90,7 +90,7 @@ double orc_setup_g(const orc_ctx *c);
  * 2 nonlinear, 3 nonlinear_prev, 4 pressure, 5 pressure_bc, 6 pressure_bc_prev, 7 velocity_bc */
 void orc_set_state(orc_ctx *c, int field, const double *data);
 void orc_get_state(const orc_ctx *c, int field, double *data);
-/* One iteration of Simulation::run's serial body (simulation.cpp:235-247); it>=1. */
+/* One iteration of Simulation::run's serial body (simulation.cpp:173-191); it>=1. */
 void orc_step(orc_ctx *c, int it);
 void orc_run(orc_ctx *c, int nsteps);
 /* Constant body force added to every step's nonlinear term (no reference counterpart). */

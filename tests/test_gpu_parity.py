@@ -112,10 +112,32 @@ def test_setup_tables(gpu, name):
         assert maxrel(a, b) < 1e-11, t
 
 
+# Wall conditions a_V u + b_V du/dy = bc per component (eqsolve_pv.cpp:16-60 closure rows), as
+# tests/test_eqsolve.cpp:138-184 varies them: Dirichlet everywhere; Neumann v (a = 0, b = 1: the
+# closure's a == b == 0 branch is the pressure's, the a_V[1] == 0 root case of the influence
+# responses); Robin on every component; u and w with different conditions, which keeps the
+# complex influence tables in the step instead of the compact real ones.
+WALLS = {
+    "dirichlet": dict(a_V=(1.0, 1.0, 1.0), b_V=(0.0, 0.0, 0.0)),
+    "neumann_v": dict(a_V=(1.0, 0.0, 1.0), b_V=(0.0, 1.0, 0.0)),
+    "robin": dict(a_V=(1.0, 1.0, 1.0), b_V=(0.05, 0.02, 0.03)),
+    "mixed_uw": dict(a_V=(1.0, 1.0, 0.5), b_V=(0.0, 0.0, 0.1)),
+}
+
+
+def _with_walls(cfg, walls, nxs=None):
+    cfg = dict(cfg, a_V=list(WALLS[walls]["a_V"]), b_V=list(WALLS[walls]["b_V"]))
+    if nxs is not None:
+        cfg["NXS"] = nxs
+    return cfg
+
+
+@pytest.mark.parametrize("walls,nxs", [("dirichlet", None), ("neumann_v", None), ("robin", None),
+                                       ("mixed_uw", None), ("neumann_v", 7), ("robin", 8), ("mixed_uw", 10)])
 @pytest.mark.parametrize("correct", [False, True])
 @pytest.mark.parametrize("fuse", [False, True])
-def test_solve_modes(gpu, correct, fuse):
-    cfg = stage3_config()
+def test_solve_modes(gpu, correct, fuse, walls, nxs):
+    cfg = _with_walls(stage3_config(), walls, nxs)
     dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
     dev.setup(150.0)
     ref.setup(150.0)
@@ -135,6 +157,71 @@ def test_solve_modes(gpu, correct, fuse):
     P2, V2 = ref.solve_modes(None if fuse else rp, bp, rv, bv, correct)
     assert maxrel(P1, P2) < 1e-10
     assert maxrel(V1, V2) < 1e-10
+
+
+@pytest.mark.parametrize("walls", ["neumann_v", "robin", "mixed_uw"])
+@pytest.mark.parametrize("name", ["channel_32", "stage3"])
+def test_setup_tables_walls(gpu, name, walls):
+    cfg = _with_walls(stage3_config() if name == "stage3" else CONFIGS[name], walls)
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    g = 1.5 / cfg["DT"] * cfg["Re"]
+    dev.setup(g)
+    ref.setup(g)
+    for t in pyoracle.Oracle.TABLES:
+        a, b = dev.table(t), ref.table(t)
+        assert maxrel(a, b) < 1e-11, t
+
+
+@pytest.mark.parametrize("name,walls,fft", [("channel_32", "robin", "register"),
+                                            ("channel_32", "mixed_uw", "register"),
+                                            ("stage3", "robin", "register"),
+                                            ("stage3", "mixed_uw", "register"),
+                                            ("channel_32", "robin", "stockham")])
+def test_step_parity_walls(gpu, monkeypatch, name, walls, fft):
+    """Five steps with Robin walls, and with u and w under different conditions, against the
+    oracle (1e-10 rel-L2 per component).  Neumann v is covered by the solve and table tests
+    only, as in the reference (tests/test_eqsolve.cpp:138-184): stepped, the oracle's field grows
+    by 1e10-1e30 in five steps under that wall condition, so a step comparison there measures
+    the growth."""
+    _use_fft(monkeypatch, fft)
+    cfg = _with_walls(stage3_config() if name == "stage3" else CONFIGS[name], walls)
+    u0 = fields.seeded_velocity(cfg, seed=6)
+    dev, ref = gpu.Channel(cfg), pyoracle.Oracle(cfg)
+    dev.set_state("velocity", u0)
+    ref.set_state("velocity", u0)
+    dev.run(5, it_begin=1)
+    ref.run(5)
+    errs = fields.rel_l2_components(dev.get_state("velocity"), ref.get_state("velocity"))
+    print(f"{name} {walls} {fft}: 5 steps rel-L2 per component {errs}")
+    assert max(errs) < 1e-10, errs
+    assert fields.rel_l2(dev.get_state("pressure"), ref.get_state("pressure")) < 1e-9
+    fd, fr = dev.flow_gradients(), ref.flow_gradients()
+    scale = 1.0 / abs(ref.table("q_unit")[0]) if cfg["INDEX_GRAD_PRE"] != 1 else 1.0
+    assert np.all(np.abs(fd - fr) <= 1e-12 * scale), (fd, fr, scale)
+
+
+def test_steps_after_an_adjoint_step_of_iteration_1(gpu):
+    """cdns_adjoint_step(it=1) sets the solver tables up for g1 = Re/dt; the forward steps that
+    follow (iterations 3..5, g2 = 1.5 Re/dt) must set their own tables up again."""
+    cfg = CONFIGS["channel_32"]
+    u0 = fields.seeded_velocity(cfg, seed=9)
+    ref = pyoracle.Oracle(cfg)
+    ref.set_state("velocity", u0)
+    ref.run(5)
+    dev = gpu.Channel(cfg)
+    dev.set_state("velocity", u0)
+    dev.run(2, it_begin=1)
+    state = {n: dev.get_state(n) for n in ("velocity", "velocity_prev", "nonlinear_prev", "pressure_bc_prev",
+                                            "pressure_bc", "pressure")}
+    rng = np.random.default_rng(3)
+    L = [rng.standard_normal(state[n].shape) + 0j for n in ("velocity", "velocity_prev", "nonlinear_prev",
+                                                              "pressure_bc_prev")]
+    dev.adjoint_step(1, u0, L, None)
+    for n, v in state.items():
+        dev.set_state(n, v)
+    dev.run(3, it_begin=3)
+    errs = fields.rel_l2_components(dev.get_state("velocity"), ref.get_state("velocity"))
+    assert max(errs) < 1e-10, errs
 
 
 @pytest.mark.parametrize("name,steps,tol,fft", [("channel_32", 1, 1e-11, "register"),
@@ -170,7 +257,7 @@ def test_step_parity(gpu, monkeypatch, name, steps, tol, fft):
     assert np.abs(pd - pr).max() <= 10 * tol * max(np.abs(pr).max(), terms * 1e-3), "pressure_bc_prev"
     assert fields.rel_l2(dev.get_state("pressure"), ref.get_state("pressure")) < 10 * tol
     # constant-flux dpdx = (Q_target - Q(u*)) / Q(u_G) cancels, so its error is measured against
-    # the scale 1/Q(u_G) of one unit of flow-rate error (simulation.cpp:187-188)
+    # the scale 1/Q(u_G) of one unit of flow-rate error (simulation.cpp:124-125)
     fd, fr = dev.flow_gradients(), ref.flow_gradients()
     scale = 1.0 / abs(ref.table("q_unit")[0]) if cfg["INDEX_GRAD_PRE"] != 1 else 1.0
     assert np.all(np.abs(fd - fr) <= 1e-12 * scale), (fd, fr, scale)

@@ -73,7 +73,7 @@ def test_step_keeps_divergence_free_no_slip_fields():
 
 def test_history_buffers_follow_the_reference_swaps():
     """After a step: velocity_prev = u^n, nonlinear_prev = N(u^n), pressure_bc_prev = B^n
-    (simulation.cpp:222-232)."""
+    (simulation.cpp:161-170)."""
     cfg = config.channel_32()
     o = pyoracle.Oracle(cfg)
     u0 = fields.seeded_velocity(cfg, seed=8)
