@@ -52,6 +52,16 @@ def algorithmic_bytes(cfg, esz):
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload
 # (FP64), each scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0); profiles/step_retau180_hbm_traffic.md holds the runs.
+# y stage of the step when CDNS_YSTAGE is not set: the matrix-core apply ("mfma") or the DCT-I
+# through FFTs of length 2 NY2 ("dct"); the bench times the other one beside it.
+DEFAULT_YSTAGE = "mfma"
+
+# SQ_INSTS_VALU of one x-product launch on this workload (FP64; rocprofv3 --pmc) and the issue
+# peak it is set against: a SIMD is 16 lanes wide, so it issues one wave64 vector instruction
+# per 4 cycles; 1024 SIMDs at 2.4 GHz.
+MEASURED_VALU = {"x_product": 44.4e6}
+VALU_PEAK_WAVE_INSTS = 1024 * 2.4e9 / 4
+
 MEASURED_TRAFFIC = {
     "x_product": 356.3e6,
     "solve_step": 374.4e6,
@@ -140,6 +150,117 @@ def counter_run(cdns, cfg, u0):
     print(json.dumps({"counter_run": "done", "steps": 5, "copy_bytes": x.numel() * 8}), flush=True)
 
 
+def run_form(cdns, cfg, u0, prec, steps, warm, ystage=None):
+    """One precision form of the step: a profiled pass (HIP events around every launch, which
+    also brings the GPU to its clocks), then W warm-up steps and K timed steps from the same
+    initial state.  Returns the timed seconds, the per-kernel (launches, ms) and the final state."""
+    import numpy as np
+    import torch
+
+    old = os.environ.get("CDNS_YSTAGE")
+    if ystage is not None:
+        os.environ["CDNS_YSTAGE"] = ystage
+    try:
+        dev = cdns.Channel(cfg, precision=prec)
+    finally:
+        if ystage is not None:
+            if old is None:
+                os.environ.pop("CDNS_YSTAGE", None)
+            else:
+                os.environ["CDNS_YSTAGE"] = old
+    dev.set_state("velocity", u0)
+    dev.run(warm, it_begin=1)
+    dev.profile(True)
+    dev.profile_reset()
+    dev.run(steps, it_begin=warm + 1)
+    dev.sync()
+    prof = dev.profile_read()
+    dev.profile(False)
+    dev.set_state("velocity", u0)
+    for n in ("velocity_prev", "nonlinear_prev"):
+        dev.set_state(n, dev.vector())
+    dev.set_state("pressure_bc_prev", dev.boundary())
+    dev.run(warm, it_begin=1)
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.run(steps, it_begin=warm + 1)
+    dev.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    final = dev.get_state("velocity")
+    dev.close()
+    return dict(elapsed=el, prof=prof, finite=bool(np.isfinite(final).all()), final=final)
+
+
+def precision_100(cdns, fields, cfg, u0):
+    """Mixed against FP64 after 1 and 100 steps from the same initial state (untimed)."""
+    out = {}
+    states = {}
+    for prec in (cdns.FP64, cdns.MIXED):
+        dev = cdns.Channel(cfg, precision=prec)
+        dev.set_state("velocity", u0)
+        dev.run(1, it_begin=1)
+        s1 = dev.get_state("velocity")
+        dev.run(99, it_begin=2)
+        states[prec] = (s1, dev.get_state("velocity"))
+        dev.close()
+    for i, n in ((0, "after_1_step"), (1, "after_100_steps")):
+        out[n] = fields.precision_report(cfg, states[cdns.MIXED][i], states[cdns.FP64][i])
+    return out
+
+
+def c5_adjoint(cdns, config, fields):
+    """The larger case (256x193x192, product 384x289x288): steps/s of the mixed and FP64 forms,
+    and the FP64 gradient of the terminal mean wall shear over 500 steps with a checkpoint every
+    25 (forward horizon + checkpointed backward) against the 500-step FP64 forward; device memory
+    of the context and of the horizon storage."""
+    import numpy as np
+    import torch
+
+    cfg = dict(config.retau395(), DT=5e-5)
+    nsteps, kc = 500, 25
+    u0 = fields.seeded_velocity(cfg, seed=395, mean="log", spectrum="k53", amplitude=0.1, kcut=16)
+    res = {"workload": "256x193x192 (product 384x289x288), Re=395, dt=5e-5, 500 steps, checkpoint every 25, "
+                       "d(mean wall shear at T)/d(u0)"}
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    for prec, name in ((cdns.MIXED, "mixed"), (cdns.FP64, "f64")):
+        dev = cdns.Channel(cfg, precision=prec)
+        dev.set_state("velocity", u0)
+        dev.run(2, it_begin=1)
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.run(nsteps - 2, it_begin=3)
+        dev.sync()
+        res[f"{name}_steps_per_s"] = (nsteps - 2) / (time.perf_counter() - t0)
+        if prec == cdns.FP64:
+            free1, _ = torch.cuda.mem_get_info()
+            res["context_bytes_f64"] = free0 - free1
+            dev.set_state("velocity", u0)
+            t0 = time.perf_counter()
+            dev.run(nsteps, it_begin=1)
+            dev.sync()
+            t_fwd = time.perf_counter() - t0
+            dev.adjoint_run(u0, None, nsteps, kc, 1)  # allocates the horizon storage once
+            free2, _ = torch.cuda.mem_get_info()
+            t0 = time.perf_counter()
+            J, gu, _ = dev.adjoint_run(u0, None, nsteps, kc, 1)
+            t_adj = time.perf_counter() - t0
+            g = dev.cfg
+            S = (2 * g["NZS"] + 1) * (g["NXS"] + 1) * ((g["NYS"] + 1 + 7) // 8 * 8)
+            Z = (g["NY2"] + 1) * g["NZ2"] * (g["NXS"] + 1)
+            nq = (2 * g["NZS"] + 1) * (g["NXS"] + 1)
+            res["horizon_storage_bytes"] = (-(-nsteps // kc) * (9 * S + 2 * nq) + kc * 6 * Z) * 16
+            res["adjoint_workspace_bytes"] = free1 - free2
+            res["forward_s"] = t_fwd
+            res["gradient_s"] = t_adj
+            res["adjoint_forward_ratio"] = (t_adj - t_fwd) / t_fwd
+            res["finite"] = bool(np.isfinite(J) and np.isfinite(gu).all())
+        dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +270,7 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the other y stage and the c5 adjoint case")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--counters", action="store_true",
                     help="no timing: run 5 FP64 steps and three 256 MiB device copies (the calibration "
@@ -158,7 +280,6 @@ def main():
         print(json.dumps({"error": "this backend runs one GPU per process; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    import numpy as np
     import torch
 
     import cdns
@@ -172,32 +293,18 @@ def main():
         counter_run(cdns, cfg, u0)
         return
 
+    ystage = os.environ.get("CDNS_YSTAGE") or DEFAULT_YSTAGE
     res = {}
     for prec, name in ((cdns.FP64, "f64"), (cdns.MIXED, "mixed")):
-        dev = cdns.Channel(cfg, precision=prec)
-        dev.set_state("velocity", u0)
-        dev.run(warm, it_begin=1)
-        dev.sync()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        dev.run(args.steps, it_begin=warm + 1)
-        dev.sync()
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        final = dev.get_state("velocity")
-        finite = bool(np.isfinite(final).all())
-        # per-kernel times: the same steps again from the same state, with HIP events on the
-        # backend's stream around every launch
-        dev.set_state("velocity", u0)
-        dev.run(warm, it_begin=1)
-        dev.profile(True)
-        dev.profile_reset()
-        dev.run(args.steps, it_begin=warm + 1)
-        dev.sync()
-        prof = dev.profile_read()
-        dev.profile(False)
-        res[name] = dict(elapsed=el, prof=prof, finite=finite, final=final)
-        dev.close()
+        res[name] = run_form(cdns, cfg, u0, prec, args.steps, warm, ystage)
+    other = {}
+    if not args.no_compare:
+        alt = "mfma" if ystage == "dct" else "dct"
+        for prec, name in ((cdns.FP64, "f64"), (cdns.MIXED, "mixed")):
+            r = run_form(cdns, cfg, u0, prec, args.steps, warm, alt)
+            other[name] = {"ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                           "kernels_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in r["prof"].items()}}
+        other["y_stage"] = alt
 
     r = res["f64"]
     steps_per_s = args.steps / r["elapsed"]
@@ -214,13 +321,22 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dk["achieved_GBs"] / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dom),
             "algorithmic_bytes_per_launch": algo[dom], "ms_per_launch": dk["ms_per_launch"]}
+    if "x_product" in kern and "x_product" in MEASURED_VALU:
+        # the resource that binds the x product: vector-ALU issue (wave-instructions per launch from
+        # SQ_INSTS_VALU against the issue peak)
+        xi = MEASURED_VALU["x_product"]
+        rate = xi / (kern["x_product"]["ms_per_launch"] * 1e-3)
+        kern["x_product"]["valu"] = {"wave_insts_per_launch": xi, "achieved_G_per_s": rate / 1e9,
+                                     "peak_G_per_s": VALU_PEAK_WAVE_INSTS / 1e9, "frac": rate / VALU_PEAK_WAVE_INSTS}
     adj = None if args.no_adjoint else adjoint_timing()
+    c5 = None if (args.no_adjoint or args.no_compare) else c5_adjoint(cdns, config, fields)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, u0, args.cpu_seconds)
     mixed_sps = args.steps / res["mixed"]["elapsed"]
     # the mixed run against the FP64 run from the same initial state after warm + steps steps
     mixed_err = fields.precision_report(cfg, res["mixed"]["final"], r["final"])
+    prec100 = precision_100(cdns, fields, cfg, u0)
     line = {
         "metric": "steps/s at 128x129x128 (product grid 196x193x196), FP64 step on one MI355X",
         "value": steps_per_s,
@@ -236,15 +352,18 @@ def main():
         "data": "synthetic: seeded log-law mean + divergence-free k^-5/3 perturbation (rms 10% of centreline)",
         "config": {"workload": f"{args.workload}: base {cfg['NX']}x{cfg['NY'] + 1}x{cfg['NZ']}, product "
                                f"{cfg['NX2']}x{cfg['NY2'] + 1}x{cfg['NZ2']}, Re={cfg['Re']}, dt={cfg['DT']}",
-                   "steps_timed_from_iteration": warm + 1},
+                   "steps_timed_from_iteration": warm + 1, "y_stage": ystage},
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": {"steps_per_s": mixed_sps, "ms_per_step": 1e3 * res["mixed"]["elapsed"] / args.steps,
                   "speedup_vs_f64": mixed_sps / steps_per_s, "finite": res["mixed"]["finite"],
                   "error_vs_f64_after_steps": warm + args.steps, "error_vs_f64": mixed_err,
+                  "error_vs_f64_1_and_100_steps": prec100,
                   "kernels_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in res["mixed"]["prof"].items()}},
+        "y_stage_alternative": other or None,
         "adjoint_forward_ratio": None if adj is None else adj["backward_pass_over_forward"],
         "adjoint": adj,
+        "adjoint_c5": c5,
         "kernels": kern,
         "finite": r["finite"],
     }

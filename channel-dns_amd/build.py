@@ -5,7 +5,8 @@ import sys
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["api.hip", "transforms.hip", "transforms_rf.hip", "transforms_y.hip", "spectral.hip", "solver.hip", "adjoint.hip", "diagnostics.hip"]
+SOURCES = ["api.hip", "transforms.hip", "transforms_rf.hip", "transforms_y.hip", "spectral.hip", "solver.hip", "adjoint.hip", "diagnostics.hip",
+           "transforms_dct.hip"]
 OUT = os.path.join(HERE, "libcdns_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
@@ -47,6 +48,14 @@ def build(verbose=False, jobs=None):
         sys.stderr.write(log)
         raise RuntimeError("hipcc link failed")
     os.replace(tmp, OUT)
+    # the C++ mirror of the reference's Simulation loop over the C ABI (host/run_steps)
+    exe = os.path.join(HERE, "host", "run_steps")
+    rc, log = _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+                    "-I", os.path.join(os.path.dirname(HERE), "include"), os.path.join(HERE, "host", "run_steps.cpp"),
+                    "-L", HERE, "-lcdns_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe])
+    if rc != 0:
+        sys.stderr.write(log)
+        raise RuntimeError("g++ failed on host/run_steps.cpp")
     return OUT
 
 

@@ -129,8 +129,13 @@ struct cdns_ctx {
     int KT = 13, P = 4;
     int rf = 1; // register-resident FFTs (CDNS_FFT=stockham selects the LDS Stockham kernels)
     int ymfma = 1; // y stages on the matrix cores (CDNS_YSTAGE=vector selects the vector-ALU kernels)
+    // y stages as a DCT-I through FFTs of length 2 NY2 (CDNS_YSTAGE=dct), with its root table
+    bool ydct = false;
+    void *ydct_tw = nullptr;
     int ycurl = 1; // curl fused into the matrix-core y backward of the step (CDNS_YCURL=0: separate)
-    int xpair = 1; // x-stage product FFT rows on lane pairs (CDNS_XPRODUCT=single keeps one lane per row)
+    // x-stage product FFT rows: 1 on lane pairs, 2 one lane per row with staged row writes
+    // (CDNS_XPRODUCT=lane), 0 one lane per row in one-wavefront blocks (CDNS_XPRODUCT=single)
+    int xpair = 1;
     int zpair = 1; // z-stage FFT rows on lane pairs (CDNS_ZSTAGE=single keeps one lane per row)
     std::vector<double> hWAVX, hWAVZ;
 
@@ -275,6 +280,24 @@ YArgs yargs(cdns_ctx *c) {
     a.mfma = c->ymfma;
     return a;
 }
+DctArgs dct_args(cdns_ctx *c) {
+    DctArgs a{};
+    const Geometry &g = c->g;
+    a.y = c->Ybuf;
+    a.YS = g.YS;
+    a.tw = c->ydct_tw;
+    a.wavx = c->WAVX;
+    a.wavz = c->WAVZ;
+    a.Re = c->cfg.Re;
+    a.ny = g.NYS;
+    a.JS = g.JS;
+    a.NQ = g.NQ;
+    a.nzk = g.nzk;
+    a.NY2 = g.NY2;
+    a.L = 2 * g.NY2;
+    a.CH = g.E / 2;
+    return a;
+}
 ZArgs zargs(cdns_ctx *c, const void *in, void *out, size_t in_fs, size_t out_fs) {
     ZArgs a{};
     a.in = in;
@@ -319,9 +342,15 @@ void backward_z(cdns_ctx *c, int n, void *zout = nullptr) {
     launch_zbwd(zargs(c, c->Ybuf, zout ? zout : c->Zbuf, c->g.YS, c->g.ZS), n, c->fp32, c->st);
 }
 void backward_yz(cdns_ctx *c, const zd *const *planes, int n, void *zout = nullptr) {
-    YArgs a = yargs(c);
-    for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
-    {
+    if (c->ydct) {
+        DctArgs a = dct_args(c);
+        for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
+        a.nf = n;
+        Launch L(c, "y_backward");
+        if (!launch_ybwd_dct(a, c->fp32, c->st)) throw std::logic_error("no DCT y stage for this NY2");
+    } else {
+        YArgs a = yargs(c);
+        for (int f = 0; f < n; ++f) a.spec[f] = planes[f];
         Launch L(c, "y_backward");
         launch_ybwd(a, n, c->fp32, c->st);
     }
@@ -332,6 +361,17 @@ void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr
     {
         Launch L(c, "z_forward");
         launch_zfwd(zargs(c, zin ? zin : c->ZNbuf, c->Ybuf, c->g.ZS, c->g.YS), n, c->fp32, c->st);
+    }
+    if (c->ydct) {
+        DctArgs a = dct_args(c);
+        for (int f = 0; f < n; ++f) {
+            a.spec_out[f] = planes[f];
+            a.add[f] = add ? add[f] : nullptr;
+        }
+        a.nf = n;
+        Launch L(c, "y_forward");
+        if (!launch_yfwd_dct(a, c->fp32, c->st)) throw std::logic_error("no DCT y stage for this NY2");
+        return;
     }
     YArgs a = yargs(c);
     a.coef = c->ycoef_fe;
@@ -543,6 +583,26 @@ void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
 // TimeStepper::enqueue_nonlinear_term on the context's velocity planes
 void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
     zd *const *add = c->body_on ? c->body : nullptr;
+    if (c->ydct && c->ycurl) {
+        // DCT-I y backward of u with the curl formed per column: the six Y fields and the wall
+        // condition
+        DctArgs a = dct_args(c);
+        for (int k = 0; k < 3; ++k) a.spec[k] = u[k];
+        a.nf = 6;
+        a.curl = 1;
+        a.bcw = c->bcw;
+        {
+            Launch L(c, "y_backward_curl");
+            if (!launch_ybwd_dct(a, c->fp32, c->st)) throw std::logic_error("no DCT y stage for this NY2");
+        }
+        backward_z(c, 6);
+        {
+            Launch L(c, "x_product");
+            launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
+        }
+        forward_zy(c, nl, 3, nullptr, add);
+        return;
+    }
     if (c->ymfma && c->ycurl) {
         // y backward of u with the curl fused in: the six Y fields and the wall condition
         YArgs a = yargs(c);
@@ -1122,9 +1182,13 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
             throw std::invalid_argument("precision must be CDNS_FP64 or CDNS_MIXED");
         c->fp32 = cfg->precision == CDNS_MIXED;
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
-        if (const char *e = getenv("CDNS_YSTAGE")) c->ymfma = std::string(e) != "vector";
+        if (const char *e = getenv("CDNS_YSTAGE")) {
+            c->ymfma = std::string(e) != "vector";
+            c->ydct = std::string(e) == "dct";
+        }
         if (const char *e = getenv("CDNS_YCURL")) c->ycurl = std::string(e) != "0";
-        if (const char *e = getenv("CDNS_XPRODUCT")) c->xpair = std::string(e) != "single";
+        if (const char *e = getenv("CDNS_XPRODUCT"))
+            c->xpair = std::string(e) == "single" ? 0 : (std::string(e) == "lane" ? 2 : 1);
         if (const char *e = getenv("CDNS_ZSTAGE")) c->zpair = std::string(e) != "single";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;
@@ -1199,6 +1263,8 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         };
         C->twz = up_tw(g.NZ2);
         C->twx = up_tw(g.NX2);
+        C->ydct = C->ydct && g.nj <= 256 && dct_supported(2 * g.NY2);
+        if (C->ydct) C->ydct_tw = up_tw(2 * g.NY2);
         // z lines per workgroup: at most 4 butterflies per thread per stage
         auto ipt = [](int lines, const FftPlan &p) { return (lines * (p.n / plan_min_radix(p)) + 255) / 256; };
         int T = (g.nkx + 15) / 16;

@@ -26,6 +26,23 @@ struct YArgs {
     zd *bcw;
 };
 
+// y stages as a DCT-I through complex FFTs of length L = 2 NY2 (transforms_dct.hip).
+struct DctArgs {
+    const zd *spec[6]; // backward inputs (curl: u, v, w)
+    zd *spec_out[6];   // forward outputs
+    const zd *add[6];  // forward: planes added to the outputs, or null
+    void *y;           // Y fields, field f at y + f*YS
+    size_t YS;
+    const void *tw;    // L roots exp(-2 pi i k / L), element type by precision
+    const double *wavx, *wavz;
+    double Re;
+    zd *bcw;           // curl: the curl part of the pressure wall condition [q][2], or null
+    int ny, JS, NQ, nzk, NY2, L, nf, curl, CH;
+};
+bool dct_supported(int L);
+bool launch_ybwd_dct(const DctArgs &a, bool fp32, hipStream_t st);
+bool launch_yfwd_dct(const DctArgs &a, bool fp32, hipStream_t st);
+
 // z stages (FFT over kz).
 struct ZArgs {
     const void *in;

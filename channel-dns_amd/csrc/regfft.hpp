@@ -60,6 +60,15 @@ template <int K, int N, int S> struct Tw {
     static constexpr double s = S * ct::trig(K, N, 1);
 };
 
+// Ordering of one wavefront's LDS accesses: a lane group's transpose buffer belongs to one
+// wavefront, whose LDS instructions execute in program order, so the FFT passes need only a
+// compiler-level fence, not a workgroup barrier that would make the other wavefronts wait.
+__device__ inline void lsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int I> using ic = std::integral_constant<int, I>;
 template <int B, int E, typename F> __device__ inline void sfor(F &&f) {
     if constexpr (B < E) {

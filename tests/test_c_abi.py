@@ -14,7 +14,7 @@ LIBDIR = os.path.join(ROOT, "channel-dns_amd")
 
 def _declared_entry_points():
     with open(os.path.join(ROOT, "include", "cdns.h")) as f:
-        text = f.read()
+        text = re.sub(r"/\*.*?\*/", "", f.read(), flags=re.S)
     return sorted(set(re.findall(r"\b(cdns_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -31,3 +31,13 @@ def test_c_caller_compiles_links_and_runs(tmp_path):
         table = f.read()
     missing = [n for n in _declared_entry_points() if not re.search(rf"\b{n}\b", table.split("cdns_api api")[1])]
     assert not missing, f"entry points of cdns.h not in the typed table: {missing}"
+
+
+def test_cpp_simulation_binding_is_built():
+    """channel-dns_amd/host/run_steps (the C++ Simulation loop over the ABI, built by build.py with
+    -Wall -Wextra -Werror) exists and rejects a wrong command line before touching a device."""
+    exe = os.path.join(LIBDIR, "host", "run_steps")
+    if not os.path.exists(exe):
+        pytest.skip("not built")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

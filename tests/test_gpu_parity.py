@@ -27,8 +27,9 @@ ODD_NY2 = config.make(32, 32, 32, 16, 30, 16, 48, 47, 48, 4 * math.pi, 2 * math.
 # Two kernel families per stage: the defaults (register-resident two-pass FFTs for x and z,
 # matrix-core y stages with the curl fused into the y backward) and the alternates (LDS Stockham
 # FFTs, CDNS_FFT=stockham; vector-ALU y stages, CDNS_YSTAGE=vector; the curl as its own kernel
-# before the matrix-core y backward, CDNS_YCURL=0).
-FFT_KINDS = ["register", "stockham", "single-lane-rows", "separate-curl"]
+# before the matrix-core y backward, CDNS_YCURL=0), and the y stages as a DCT-I through FFTs of
+# length 2 NY2 (CDNS_YSTAGE=dct; meshes whose 2 NY2 has no instantiation keep the matrix cores).
+FFT_KINDS = ["register", "stockham", "single-lane-rows", "separate-curl", "dct"]
 
 
 def _use_fft(monkeypatch, kind):
@@ -36,6 +37,8 @@ def _use_fft(monkeypatch, kind):
         monkeypatch.delenv(v, raising=False)
     if kind == "separate-curl":
         monkeypatch.setenv("CDNS_YCURL", "0")
+    if kind == "dct":
+        monkeypatch.setenv("CDNS_YSTAGE", "dct")
     if kind == "stockham":
         monkeypatch.setenv("CDNS_FFT", "stockham")
         monkeypatch.setenv("CDNS_YSTAGE", "vector")
@@ -113,10 +116,21 @@ def test_setup_tables(gpu, name):
 
 
 # Wall conditions a_V u + b_V du/dy = bc per component (eqsolve_pv.cpp:16-60 closure rows), as
-# tests/test_eqsolve.cpp:138-184 varies them: Dirichlet everywhere; Neumann v (a = 0, b = 1: the
-# closure's a == b == 0 branch is the pressure's, the a_V[1] == 0 root case of the influence
-# responses); Robin on every component; u and w with different conditions, which keeps the
-# complex influence tables in the step instead of the compact real ones.
+# tests/test_eqsolve.cpp:138-184 varies them: Dirichlet everywhere; Neumann v (a = 0, b = 1, with
+# the a_V[1] == 0 root case of the influence normalisation); Robin on every component; u and w
+# with different conditions, which keeps the complex influence tables in the step instead of the
+# compact real ones.
+#
+# Under Neumann v with Dirichlet u, w the wall divergence i kx u + i kz w + dv/dy is the wall
+# value of dv/dy itself, whatever the wall pressures, so the 2x2 influence matrix of every mode
+# but the mean one is singular: its normalised entries are round-off divided by round-off (the
+# oracle gives 3e16 on the 32^3 mesh), and everything downstream of them (the corrected solve,
+# the B problem, sigma) is not a quantity two implementations can agree on.  The reference's own
+# test only compares two orderings of the same arithmetic.  For Neumann v the comparison is
+# therefore restricted to what the wall condition defines: the tau solves behind the raw
+# responses, the mean-mode entries and the mean mode of a corrected solve.
+NEUMANN_DEFINED_TABLES = ["wn_p", "wn_v", "ad_p", "ad_v", "P_infl0", "P_infl1", "V_infl0", "V_infl1", "u_dpdx",
+                          "w_dpdz", "q_unit"]
 WALLS = {
     "dirichlet": dict(a_V=(1.0, 1.0, 1.0), b_V=(0.0, 0.0, 0.0)),
     "neumann_v": dict(a_V=(1.0, 0.0, 1.0), b_V=(0.0, 1.0, 0.0)),
@@ -155,6 +169,10 @@ def test_solve_modes(gpu, correct, fuse, walls, nxs):
     bv[...] = np.sin(i * .31) * .001 + 1j * np.cos(i * .37) * .002
     P1, V1 = dev.solve_modes(None if fuse else rp, bp, rv, bv, correct)
     P2, V2 = ref.solve_modes(None if fuse else rp, bp, rv, bv, correct)
+    if walls == "neumann_v":
+        Z = cfg["NZS"]  # the mean mode only (see NEUMANN_DEFINED_TABLES)
+        P1, P2, V1, V2 = P1[Z, :, 0], P2[Z, :, 0], V1[Z, :, 0, :], V2[Z, :, 0, :]
+        assert np.isfinite(P2).all() and np.abs(P2).max() > 0
     assert maxrel(P1, P2) < 1e-10
     assert maxrel(V1, V2) < 1e-10
 
@@ -167,9 +185,16 @@ def test_setup_tables_walls(gpu, name, walls):
     g = 1.5 / cfg["DT"] * cfg["Re"]
     dev.setup(g)
     ref.setup(g)
-    for t in pyoracle.Oracle.TABLES:
+    # the B problem and sigma are outputs of a corrected solve (sigma up to 1e2-1e4 here): 1e-10
+    tables = NEUMANN_DEFINED_TABLES if walls == "neumann_v" else pyoracle.Oracle.TABLES
+    for t in tables:
         a, b = dev.table(t), ref.table(t)
-        assert maxrel(a, b) < 1e-11, t
+        assert maxrel(a, b) < 1e-10, t
+    if walls == "neumann_v":
+        Z = cfg["NZS"]
+        for t in ("infl_div0", "infl_div1"):  # the root entries: (1, 0) for a_V[1] = 0
+            a, b = dev.table(t)[Z, :, 0], ref.table(t)[Z, :, 0]
+            assert np.array_equal(b, np.array([1.0, 0.0])) and maxrel(a, b) < 1e-15, t
 
 
 @pytest.mark.parametrize("name,walls,fft", [("channel_32", "robin", "register"),
@@ -232,6 +257,9 @@ def test_steps_after_an_adjoint_step_of_iteration_1(gpu):
                                                 ("stage3", 5, 1e-10, "stockham"),
                                                 ("channel_32", 5, 1e-10, "single-lane-rows"),
                                                 ("channel_32", 5, 1e-10, "separate-curl"),
+                                                ("channel_32", 20, 1e-9, "dct"),
+                                                ("stage3", 5, 1e-10, "dct"),
+                                                ("channel_64", 5, 1e-10, "dct"),
                                                 ("odd_ny2", 5, 1e-10, "register"),
                                                 ("odd_ny2", 5, 1e-10, "separate-curl"),
                                                 ("odd_ny2", 5, 1e-10, "stockham")])
@@ -285,9 +313,12 @@ def _retau180_reference(steps):
     return cfg, _RT180["u0"], _RT180[steps]
 
 
-@pytest.mark.parametrize("steps,tol", [(1, 1e-11), (20, 1e-9)])
-def test_step_parity_retau180(gpu, steps, tol):
-    """The bench workload (128x129x128, product 196x193x196): FP64 step vs the oracle."""
+@pytest.mark.parametrize("steps,tol,ystage", [(1, 1e-11, "mfma"), (20, 1e-9, "mfma"), (1, 1e-11, "dct"),
+                                              (20, 1e-9, "dct")])
+def test_step_parity_retau180(gpu, monkeypatch, steps, tol, ystage):
+    """The bench workload (128x129x128, product 196x193x196): FP64 step vs the oracle, with the y
+    stages on the matrix cores and as a DCT-I."""
+    _use_fft(monkeypatch, "dct" if ystage == "dct" else "register")
     cfg, u0, want = _retau180_reference(steps)
     dev = gpu.Channel(cfg)
     dev.set_state("velocity", u0)

@@ -33,13 +33,18 @@ def _check(gpu, label, cfg, u0, steps):
     assert max(rep["divergence_l2"]) < 1e-10 * scale, rep
 
 
-@pytest.mark.parametrize("steps", [1, 100])
-def test_mixed_retau180_reference_dt(gpu, steps):
+@pytest.mark.parametrize("steps,ystage", [(1, "mfma"), (100, "mfma"), (1, "dct"), (100, "dct")])
+def test_mixed_retau180_reference_dt(gpu, monkeypatch, steps, ystage):
     """128x129x128 (product 196x193x196), full k^-5/3 perturbation, rms 10% of the centreline,
-    at the reference's DT for this grid (5e-5)."""
+    at the reference's DT for this grid (5e-5); y stages on the matrix cores or as a DCT-I (both
+    precisions use the same y stage)."""
+    if ystage == "dct":
+        monkeypatch.setenv("CDNS_YSTAGE", "dct")
+    else:
+        monkeypatch.delenv("CDNS_YSTAGE", raising=False)
     cfg = config.retau180(dt=5e-5)
     u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
-    _check(gpu, "retau180 dt=5e-5", cfg, u0, steps)
+    _check(gpu, f"retau180 dt=5e-5 y {ystage}", cfg, u0, steps)
 
 
 def test_mixed_retau180_dt1e3(gpu):
