@@ -133,9 +133,7 @@ struct cdns_ctx {
     bool ydct = false;
     void *ydct_tw = nullptr;
     int ycurl = 1; // curl fused into the matrix-core y backward of the step (CDNS_YCURL=0: separate)
-    // x-stage product FFT rows: 1 on lane pairs, 2 one lane per row with staged row writes
-    // (CDNS_XPRODUCT=lane), 0 one lane per row in one-wavefront blocks (CDNS_XPRODUCT=single)
-    int xpair = 1;
+    int xpair = 1; // x-stage product FFT rows on lane pairs (CDNS_XPRODUCT=single keeps one lane per row)
     int zpair = 1; // z-stage FFT rows on lane pairs (CDNS_ZSTAGE=single keeps one lane per row)
     std::vector<double> hWAVX, hWAVZ;
 
@@ -1187,8 +1185,7 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
             c->ydct = std::string(e) == "dct";
         }
         if (const char *e = getenv("CDNS_YCURL")) c->ycurl = std::string(e) != "0";
-        if (const char *e = getenv("CDNS_XPRODUCT"))
-            c->xpair = std::string(e) == "single" ? 0 : (std::string(e) == "lane" ? 2 : 1);
+        if (const char *e = getenv("CDNS_XPRODUCT")) c->xpair = std::string(e) != "single";
         if (const char *e = getenv("CDNS_ZSTAGE")) c->zpair = std::string(e) != "single";
         c->esz = c->fp32 ? 8 : 16;
         g.nkx = g.NXS + 1;

@@ -251,106 +251,6 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     write_rows(3, 0);
 }
 
-// ------------------------------------------- x stage + product, one lane per row --
-// k_xprod_pair's work with each FFT row on one lane (no lane-pair butterflies: about a third fewer
-// vector instructions per transform, twice the data registers), several wavefronts per block and
-// the retained coefficients written as line-major row segments like the pair kernel.
-template <typename R> constexpr int kXLWaves = sizeof(R) == 8 ? 4 : 8;
-
-template <typename R, int N1, int N2>
-__global__ void __launch_bounds__(64 * kXLWaves<R>) k_xprod_lane(XArgs a) {
-    using Gm = Geo<N1, N2>;
-    constexpr int W = kXLWaves<R>;
-    constexpr int N = Gm::N, MX = Gm::MX, G = Gm::G, NG = Gm::NG, LB = Gm::LB;
-    constexpr int PB = W * NG;          // points per block
-    constexpr int KP = (N / 2 + G) / G; // retained kx <= K <= N/2: entries k = t + G i, i < KP
-    __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> buf[PB][LB];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
-    const int gb = wv * NG + g;
-    const int np = a.NL * a.NZ2, K = a.nkx - 1;
-    const int p0 = blockIdx.x * PB, p = p0 + gb;
-    const bool valid = p < np;
-    cpx<R> *B = buf[gb];
-    cpx<R> ph[3][MX];
-    const unsigned prow = (unsigned)(valid ? p : 0) * (unsigned)a.nkx;
-    sfor<0, 3>([&](auto line) {
-        constexpr int LN = decltype(line)::value;
-        const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]);
-        const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]);
-        sfor<0, MX>([&](auto m) {
-            constexpr int M = decltype(m)::value;
-            if constexpr (M < N2)
-                ph[LN][M] = herm_pack_nb(A, Bv, prow, t + N1 * M, N, K, valid && t < N1);
-            else
-                ph[LN][M] = cpx<R>{R(0), R(0)};
-        });
-        if constexpr (LN == 0) {
-            load_roots<R, N>(tw, a.tw);
-            __syncthreads();
-        }
-        two_pass<R, N1, N2, +1>(ph[LN], t, tw, B, lsync);
-    });
-    // u x w at the points n = t + N2 k1 held by this thread
-    cpx<R> F[MX], Fz[MX];
-    sfor<0, MX>([&](auto j) {
-        F[j] = {R(0), R(0)};
-        Fz[j] = {R(0), R(0)};
-    });
-    if (t < N2) {
-        sfor<0, N1>([&](auto k) {
-            const R u = ph[0][k].x, v = ph[0][k].y, w = ph[1][k].x, ox = ph[1][k].y, oy = ph[2][k].x, oz = ph[2][k].y;
-            F[k] = {v * oz - w * oy, w * ox - u * oz};
-            Fz[k] = {u * oy - v * ox, R(0)};
-        });
-    }
-    const R half = R(0.5) / R(N);
-    const int wj = threadIdx.x % PB, wr = threadIdx.x / PB, pw = p0 + wj;
-    const int wl = pw / a.NZ2;
-    const size_t wbase = ((size_t)wl * a.nkx) * a.NZ2 + (pw - wl * a.NZ2);
-    // forward of (Nx + i Ny) and of Nz: transposed plan, input already in place
-    two_pass<R, N2, N1, -1>(F, t, tw, B, lsync);
-    if (t < N1) sfor<0, N2>([&](auto k) { B[t + N1 * decltype(k)::value] = F[k]; });
-    lsync();
-    cpx<R> sx[KP], sy[KP], sz[KP];
-    sfor<0, KP>([&](auto i) {
-        const int k = t + G * decltype(i)::value;
-        if (k <= K) {
-            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
-            const cpx<R> s = f + gm, d = f - gm;
-            sx[i] = {half * s.x, half * s.y};
-            sy[i] = {half * d.y, -half * d.x};
-        }
-    });
-    lsync();
-    two_pass<R, N2, N1, -1>(Fz, t, tw, B, lsync);
-    if (t < N1) sfor<0, N2>([&](auto k) { B[t + N1 * decltype(k)::value] = Fz[k]; });
-    lsync();
-    sfor<0, KP>([&](auto i) {
-        const int k = t + G * decltype(i)::value;
-        if (k <= K) {
-            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
-            const cpx<R> s = f + gm;
-            sz[i] = {half * s.x, half * s.y};
-        }
-    });
-    lsync();
-    sfor<0, KP>([&](auto i) {
-        const int k = t + G * decltype(i)::value;
-        if (k <= K) {
-            B[k] = sx[i];
-            B[a.nkx + k] = sy[i];
-            B[2 * a.nkx + k] = sz[i];
-        }
-    });
-    __syncthreads();
-    if (pw < np)
-        for (int fo = 0; fo < 3; ++fo) {
-            cpx<R> *out = reinterpret_cast<cpx<R> *>(a.fout[fo]) + wbase;
-            for (int k = wr; k < a.nkx; k += 64 * W / PB) out[(size_t)k * a.NZ2] = buf[wj][fo * a.nkx + k];
-        }
-}
-
 // ------------------------------------------- single-field x stages (physical) --
 // Two points share one complex line: a + i b.
 template <typename R, int N1, int N2>
@@ -602,15 +502,6 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
 
 // ------------------------------------------------------------------ launch --
 template <typename R, int N1, int N2> static void go_xprod(const XArgs &a, hipStream_t st) {
-    if constexpr (N1 <= 16 && N2 <= 16) {
-        // one lane per row, the three retained rows of a point staged in its transpose buffer
-        if (a.pair == 2 && 3 * a.nkx <= Geo<N1, N2>::LB) {
-            constexpr int PB = kXLWaves<R> * Geo<N1, N2>::NG;
-            hipLaunchKernelGGL((k_xprod_lane<R, N1, N2>), dim3((a.NL * a.NZ2 + PB - 1) / PB), dim3(64 * kXLWaves<R>),
-                               0, st, a);
-            return;
-        }
-    }
     if constexpr (N1 % 2 == 0 && N2 % 2 == 0) {
         // the three retained rows of a point are staged in its transpose buffer
         if (a.pair && 3 * a.nkx <= PGeo<N1, N2>::LB) {
