@@ -35,7 +35,7 @@ ABI_FUNCTIONS = [
     "cdns_set_state", "cdns_get_state", "cdns_step", "cdns_run", "cdns_flow_gradients", "cdns_time",
     "cdns_profile", "cdns_profile_read", "cdns_profile_reset", "cdns_set_body_force", "cdns_objective",
     "cdns_adjoint_step", "cdns_adjoint_run", "cdns_xyzfft", "cdns_to_physical", "cdns_plane_moments",
-    "cdns_diagnostics", "cdns_engine_create", "cdns_engine_draw", "cdns_engine_destroy",
+    "cdns_diagnostics", "cdns_engine_create", "cdns_engine_draw", "cdns_engine_destroy", "cdns_adjoint_run_noalloc",
 ]
 
 
@@ -85,6 +85,7 @@ def lib(path=None):
         L.cdns_objective.argtypes = [vp, i, vp, vp, vp, i]
         L.cdns_adjoint_step.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, i]
         L.cdns_adjoint_run.argtypes = [vp, vp, vp, i, i, i, vp, vp, vp, i]
+        L.cdns_adjoint_run_noalloc.argtypes = [vp, vp, vp, i, i, i, vp, vp, vp, i]
         L.cdns_xyzfft.argtypes = [vp, ctypes.c_char, vp, vp, i, i]
         L.cdns_to_physical.argtypes = [vp, vp, vp, vp, i]
         L.cdns_plane_moments.argtypes = [vp, i, vp, vp]
@@ -97,7 +98,8 @@ def lib(path=None):
         for n in ("cdns_xyzfft2", "cdns_curl", "cdns_cross_product", "cdns_nonlinear_term", "cdns_setup",
                   "cdns_solve_modes", "cdns_get_table", "cdns_set_state", "cdns_get_state", "cdns_step",
                   "cdns_run", "cdns_flow_gradients", "cdns_profile", "cdns_profile_read", "cdns_profile_reset",
-                  "cdns_set_body_force", "cdns_objective", "cdns_adjoint_step", "cdns_adjoint_run", "cdns_xyzfft",
+                  "cdns_set_body_force", "cdns_objective", "cdns_adjoint_step", "cdns_adjoint_run",
+                  "cdns_adjoint_run_noalloc", "cdns_xyzfft",
                   "cdns_to_physical", "cdns_plane_moments", "cdns_diagnostics"):
             getattr(L, n).restype = i
         _LIB = L
@@ -350,6 +352,16 @@ class Channel:
                                            _ptr(gu), _ptr(gf) if f is not None else None, ctypes.byref(J), HOST))
         self.it = nsteps
         return J.value, gu, gf
+
+    def adjoint_run_device(self, u0, force, nsteps, k_ckpt, objective, grad_u0, grad_force, noalloc=False):
+        """cdns_adjoint_run (or, with noalloc, cdns_adjoint_run_noalloc) on device arrays (torch
+        tensors in the reference layout, complex as (re, im) pairs); returns J."""
+        fn = lib().cdns_adjoint_run_noalloc if noalloc else lib().cdns_adjoint_run
+        J = ctypes.c_double()
+        self._check(fn(self._h, _ptr(u0), _ptr(force), int(nsteps), int(k_ckpt), int(objective), _ptr(grad_u0),
+                       _ptr(grad_force), ctypes.byref(J), DEVICE))
+        self.it = nsteps
+        return J.value
 
     # per-kernel HIP-event timing
     def profile(self, enable=True):

@@ -9,12 +9,37 @@
 #include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <random>
 
 using namespace cdns;
 
+// Horizon storage of cdns_adjoint_run_noalloc: one array per GPU, held until the process ends.
+// Host code never uses the array's name as an address: cdns_create looks its device address up
+// once per device (scratch_address) and every use goes through that address.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on the first call per device).
+void *scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
 
 // ---------------------------------------------------------------- tables --
 FftPlan make_plan(int n) {
@@ -169,6 +194,10 @@ struct cdns_ctx {
     // between calls and grown when a call needs more
     void *hz_ck = nullptr, *hz_seg = nullptr;
     size_t hz_ck_bytes = 0, hz_seg_bytes = 0;
+    std::vector<double> Jhost; // host copy of the objective's per-column partial sums
+    // the device the context was created on and its address of the no-allocation scratch array
+    int device = 0;
+    void *noalloc_scratch = nullptr;
 
     // coarse grid NX x (NY+1) x NZ (Transforms::channel_xyzfft): FP64 tables and plans, built on
     // first use; the transforms run in the product-grid workspaces, which are larger
@@ -826,6 +855,7 @@ void ensure_adjoint(cdns_ctx *c) {
     c->ZL = c->alloc<unsigned char>(3 * g.ZS * c->esz);
     c->ZNa = c->alloc<unsigned char>(6 * g.ZS * c->esz);
     c->Jpart = c->alloc<double>(g.NQ);
+    c->Jhost.assign(g.NQ, 0.0);
     for (int k = 0; k < 3; ++k) c->body_save[k] = c->alloc<zd>(g.S);
     c->adj_ready = true;
 }
@@ -954,7 +984,7 @@ double objective(cdns_ctx *c, int which, zd *const *u, zd *const *grad) {
         Launch L(c, "objective");
         launch_objective(a, c->g.E, c->st);
     }
-    std::vector<double> h(c->g.NQ);
+    std::vector<double> &h = c->Jhost;
     CDNS_HIP(hipMemcpyAsync(h.data(), c->Jpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     CDNS_HIP(hipStreamSynchronize(c->st));
     double J = 0.0;
@@ -1200,6 +1230,8 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         g.YS = (size_t)g.NL * g.NQ;
         g.ZS = (size_t)g.NL * g.NZ2 * g.nkx;
         CDNS_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        CDNS_HIP(hipGetDevice(&c->device));
+        c->noalloc_scratch = scratch_address(c->device);
         cdns_ctx *C = c.get();
 
         // wavenumbers and Chebyshev weights (transforms.cpp:71-108)
@@ -1914,6 +1946,38 @@ int cdns_adjoint_run(cdns_ctx *c, const double *u0, const double *force, int nst
         }
         adjoint_horizon(c, u0, force, nsteps, k_ckpt, objective_id, grad_u0, grad_force, J, where, c->hz_ck,
                         c->hz_seg);
+        return 0;
+    } catch (const std::exception &e) {
+        if (c && c->st) (void)hipStreamSynchronize(c->st);
+        return fail(c, e);
+    }
+}
+
+int cdns_adjoint_run_noalloc(cdns_ctx *c, const double *u0, const double *force, int nsteps, int k_ckpt,
+                             int objective_id, double *grad_u0, double *grad_force, double *J, int where) {
+    try {
+        if (!c) throw std::invalid_argument("null context");
+        if (where != CDNS_DEVICE)
+            throw std::invalid_argument("cdns_adjoint_run_noalloc takes device pointers only (CDNS_DEVICE)");
+        if (c->fp32) throw std::logic_error("the discrete adjoint runs on FP64 contexts");
+        if (!c->adj_ready)
+            throw std::logic_error("cdns_adjoint_run_noalloc: the context's adjoint workspaces are not set up (an "
+                                   "earlier cdns_adjoint_run or cdns_adjoint_step on this context creates them)");
+        if (force && !c->body[0])
+            throw std::logic_error("cdns_adjoint_run_noalloc: no body-force buffer on this context (an earlier "
+                                   "cdns_set_body_force or cdns_adjoint_run with a force creates it)");
+        const HorizonSize hs = horizon_size(c, nsteps, k_ckpt);
+        const size_t ck_bytes = (hs.ck_bytes + 255) / 256 * 256;
+        if (ck_bytes + hs.seg_bytes > kNoallocScratchBytes)
+            throw std::length_error("cdns_adjoint_run_noalloc: the horizon needs " +
+                                    std::to_string(ck_bytes + hs.seg_bytes) + " bytes of storage, more than the " +
+                                    std::to_string(kNoallocScratchBytes) + " of its scratch array");
+        int dev = -1;
+        CDNS_HIP(hipGetDevice(&dev));
+        if (dev != c->device) throw std::logic_error("cdns_adjoint_run_noalloc: call it with the context's device current");
+        char *base = static_cast<char *>(c->noalloc_scratch);
+        adjoint_horizon(c, u0, force, nsteps, k_ckpt, objective_id, grad_u0, grad_force, J, where, base,
+                        base + ck_bytes);
         return 0;
     } catch (const std::exception &e) {
         if (c && c->st) (void)hipStreamSynchronize(c->st);

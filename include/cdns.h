@@ -151,6 +151,20 @@ int cdns_adjoint_step(cdns_ctx *ctx, int it, const double *velocity, double *adj
 int cdns_adjoint_run(cdns_ctx *ctx, const double *u0, const double *force, int nsteps, int k_ckpt,
                      int objective, double *grad_u0, double *grad_force, double *J, int where);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  cdns_adjoint_run with
+ * the checkpoints and the segment's fields in a file-scope device array of 64 MiB per GPU (held
+ * until the process ends) instead of memory the context allocates; it calls no hipMalloc or
+ * hipFree.  Device pointers only (where = CDNS_DEVICE), the context's device current.  It fails,
+ * doing nothing, when the horizon needs more than 64 MiB (see cdns_adjoint_run for the size; the
+ * checkpoint part is rounded up to 256 bytes), when the context's adjoint workspaces do not exist
+ * yet (an earlier cdns_adjoint_run or cdns_adjoint_step creates them) or, with a force, when no
+ * body-force buffer exists yet (cdns_set_body_force creates it).  Every call on one GPU uses the
+ * same array (every entry is written before it is read, nothing needs clearing): calls that
+ * overlap on one GPU, from any context, race. */
+int cdns_adjoint_run_noalloc(cdns_ctx *ctx, const double *u0, const double *force, int nsteps, int k_ckpt,
+                             int objective, double *grad_u0, double *grad_force, double *J, int where);
+
 /* ---- Seeded initial fields (host only, no device work). */
 
 /* initial_bc::DisturbanceEngine, which is std::mt19937_64 (src/sim/initial_bc.hpp:13), seeded

@@ -42,6 +42,8 @@ typedef struct {
     int (*engine_create)(unsigned long long, cdns_engine **);
     int (*engine_draw)(cdns_engine *, int, void *, unsigned long long);
     void (*engine_destroy)(cdns_engine *);
+    int (*adjoint_run_noalloc)(cdns_ctx *, const double *, const double *, int, int, int, double *, double *,
+                               double *, int);
 } cdns_api;
 
 static const cdns_api api = {
@@ -51,7 +53,7 @@ static const cdns_api api = {
     cdns_get_table,     cdns_set_state,     cdns_get_state,      cdns_step,          cdns_run,
     cdns_flow_gradients, cdns_time,         cdns_profile,        cdns_profile_read,  cdns_profile_reset,
     cdns_set_body_force, cdns_objective,    cdns_adjoint_step,   cdns_adjoint_run,   cdns_engine_create,
-    cdns_engine_draw,   cdns_engine_destroy,
+    cdns_engine_draw,   cdns_engine_destroy, cdns_adjoint_run_noalloc,
 };
 
 int main(void) {
