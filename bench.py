@@ -132,12 +132,12 @@ def cpu_baseline(cfg, u0, seconds_target=15.0, threads=None):
             "sample": f"{k} steps (it=3..{2 + k}) of the same workload, oracle on {n} OpenMP threads, {el:.1f} s"}
 
 
-def counter_run(cdns, cfg, u0):
-    """Steps 1-5 of the FP64 step, then a plain device copy of known size (256 MiB each way)
-    whose counter readings scale FETCH_SIZE and WRITE_SIZE to bytes."""
+def counter_run(cdns, cfg, u0, precision):
+    """Steps 1-5 of the step in one precision form, then a plain device copy of known size
+    (256 MiB each way) whose counter readings scale FETCH_SIZE and WRITE_SIZE to bytes."""
     import torch
 
-    dev = cdns.Channel(cfg)
+    dev = cdns.Channel(cfg, precision=precision)
     dev.set_state("velocity", u0)
     dev.run(5, it_begin=1)
     dev.sync()
@@ -147,7 +147,8 @@ def counter_run(cdns, cfg, u0):
         y.copy_(x)
     torch.cuda.synchronize()
     dev.close()
-    print(json.dumps({"counter_run": "done", "steps": 5, "copy_bytes": x.numel() * 8}), flush=True)
+    print(json.dumps({"counter_run": "done", "steps": 5, "precision": precision, "copy_bytes": x.numel() * 8}),
+          flush=True)
 
 
 def run_form(cdns, cfg, u0, prec, steps, warm, ystage=None):
@@ -275,6 +276,7 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="no timing: run 5 FP64 steps and three 256 MiB device copies (the calibration "
                          "of FETCH_SIZE / WRITE_SIZE) for a rocprofv3 --pmc pass")
+    ap.add_argument("--counters-mixed", action="store_true", help="the counter run in the mixed form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs one GPU per process; multi-GPU runs are not part of this build",
@@ -289,8 +291,8 @@ def main():
     cfg = dict(config.WORKLOADS[args.workload](), DT=args.dt)
     u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
     warm = max(args.warmup, 2)
-    if args.counters:
-        counter_run(cdns, cfg, u0)
+    if args.counters or args.counters_mixed:
+        counter_run(cdns, cfg, u0, cdns.MIXED if args.counters_mixed else cdns.FP64)
         return
 
     ystage = os.environ.get("CDNS_YSTAGE") or DEFAULT_YSTAGE
