@@ -123,6 +123,58 @@ template <typename R, int N, int S> struct RDft {
     }
 };
 
+// RDft for inputs that are zero from index NZ on (those entries are not read): each sub-DFT of
+// the recursion is pruned to its nonzero inputs, and the base transforms with a zero tail become
+// direct sums with compile-time twiddles.
+template <typename R, int N, int S, int NZ> struct RDftP {
+    __device__ static inline void run(cpx<R> (&v)[N]) {
+        if constexpr (NZ >= N) {
+            RDft<R, N, S>::run(v);
+        } else if constexpr (NZ <= 0) {
+            sfor<0, N>([&](auto k) { v[k] = cpx<R>{R(0), R(0)}; });
+        } else if constexpr (NZ == 1) {
+            sfor<1, N>([&](auto k) { v[k] = v[0]; });
+        } else if constexpr (N == 2 || N == 3 || N == 4 || N == 5 || N == 7 || N == 8) {
+            cpx<R> out[N];
+            sfor<0, N>([&](auto k) {
+                constexpr int K = decltype(k)::value;
+                cpx<R> acc = v[0];
+                sfor<1, NZ>([&](auto q) {
+                    constexpr int Q = decltype(q)::value;
+                    acc = acc + twmul<R, (Q * K) % N, N, S>(v[Q]);
+                });
+                out[K] = acc;
+            });
+            sfor<0, N>([&](auto k) { v[k] = out[k]; });
+        } else {
+            constexpr int p = small_factor(N), m = N / p;
+            constexpr int NZP = NZ < p ? NZ : p; // sub-transforms r < NZP have a nonzero input
+            cpx<R> sub[p][m];
+            sfor<0, NZP>([&](auto r) {
+                constexpr int RR = decltype(r)::value;
+                constexpr int NZR = (NZ - RR + p - 1) / p;
+                sfor<0, m>([&](auto j) {
+                    constexpr int J = decltype(j)::value;
+                    if constexpr (J < NZR) sub[RR][J] = v[RR + p * J];
+                    else sub[RR][J] = cpx<R>{R(0), R(0)};
+                });
+                RDftP<R, m, S, NZR>::run(sub[RR]);
+            });
+            sfor<0, m>([&](auto k) {
+                constexpr int K = decltype(k)::value;
+                cpx<R> t[p];
+                sfor<0, p>([&](auto r) {
+                    constexpr int RR = decltype(r)::value;
+                    if constexpr (RR < NZP) t[RR] = twmul<R, RR * K, N, S>(sub[RR][K]);
+                    else t[RR] = cpx<R>{R(0), R(0)};
+                });
+                RDftP<R, p, S, NZP>::run(t);
+                sfor<0, p>([&](auto qq) { v[K + m * qq] = t[qq]; });
+            });
+        }
+    }
+};
+
 // Lane-group layout of the two-pass FFT of one line of N = N1*N2 points.
 //   pass A: thread t < N1 holds x[t + N1*n2], n2 < N2; DFT over n2; times w^(t*k2); to LDS T[t][k2]
 //   pass B: thread t < N2 holds T[n1][t], n1 < N1; DFT over n1 -> X[t + N2*k1], k1 < N1
@@ -162,6 +214,38 @@ __device__ __attribute__((always_inline)) inline void two_pass(cpx<R> (&x)[M], i
     sync();
 }
 
+
+// two_pass with the pass-A input x[n2] zero for n2 >= NZA (a zero-padded line: the pass-A DFTs are
+// pruned to their nonzero inputs).
+template <typename R, int N1, int N2, int S, int NZA, int M, typename Sync>
+__device__ __attribute__((always_inline)) inline void two_pass_pruned(cpx<R> (&x)[M], int t, const cpx<R> *tw, cpx<R> *T,
+                                                                     Sync sync) {
+    constexpr int N = N1 * N2, PA = N2 + 1;
+    if (t < N1) {
+        cpx<R> a[N2];
+        sfor<0, N2>([&](auto j) {
+            constexpr int J = decltype(j)::value;
+            if constexpr (J < NZA) a[J] = x[J];
+            else a[J] = cpx<R>{R(0), R(0)};
+        });
+        RDftP<R, N2, S, NZA>::run(a);
+        sfor<0, N2>([&](auto k2) {
+            constexpr int K2 = decltype(k2)::value;
+            cpx<R> w = tw[t * K2]; // t K2 <= (N1 - 1)(N2 - 1) < N
+            if (S > 0) w.y = -w.y;
+            const cpx<R> v = a[K2];
+            T[t * PA + K2] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+        });
+    }
+    sync();
+    if (t < N2) {
+        cpx<R> b[N1];
+        sfor<0, N1>([&](auto n1) { b[n1] = T[n1 * PA + t]; });
+        RDft<R, N1, S>::run(b);
+        sfor<0, N1>([&](auto k1) { x[k1] = b[k1]; });
+    }
+    sync();
+}
 
 // ------------------------------------------------------- lane-pair DFTs (2H points) --
 // The two lanes of a pair (lane ^ 1) hold H points each of one DFT of length 2H; a radix-2

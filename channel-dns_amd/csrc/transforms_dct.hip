@@ -1,5 +1,5 @@
 // y stages as a DCT-I computed through complex FFTs of length L = 2 NY2 (register-resident
-// two-pass FFTs on lane pairs), the alternative to the matrix-core Chebyshev apply of
+// two-pass FFTs), the alternative to the matrix-core Chebyshev apply of
 // transforms_y.hip.  With N = NY2:
 //
 // Backward (transforms.cpp:692-707): y_l = sum_j c_j cos(pi j l / N), l = 0..N, for the column
@@ -10,12 +10,11 @@
 //   X_j + X_{L-j}, X the DFT of (Y_0/2, Y_1, ..., Y_{N-1}, Y_N/2, 0, ...); S_0 = S'_0 / (2N),
 //   S_j = S'_j / N for 1 <= j <= NYS.
 //
-// A workgroup owns Q consecutive columns q of one field (FP64 8, FP32 16, so that each row of the
-// Y field is written or read as a 128-byte segment); each FFT line sits on a lane group of the
-// wavefront, with each FFT row on a lane pair.  The Y side goes through an LDS tile [l][column]
-// that reuses the FFT buffers; the spectral side is read and written column-contiguous.  The
-// fields of one column group are neighbouring blocks on one XCD, so the columns they share stay
-// in that XCD's L2.
+// A workgroup owns 8 (FP64) or 16 (FP32) consecutive columns q of one field, so that each row of
+// the Y field is written or read as a 128-byte segment; each FFT line sits on a lane group of the
+// wavefront.  The Y side goes through an LDS tile [l][column] that reuses the FFT buffers; the
+// spectral side is read and written column-contiguous.  The fields of one column group are
+// neighbouring blocks on one XCD, so the columns they share stay in that XCD's L2.
 //
 // The backward with the curl (launch_ybwd_dct_curl) forms the spectral vorticity of each column
 // in FP64 before its transform: wx = D w - i kz v, wy = i kz u - i kx w, wz = i kx v - D u
@@ -29,23 +28,6 @@
 #include "wave.hpp"
 
 namespace cdns {
-
-constexpr int kDctWaves = 8;
-
-template <typename R, int N1, int N2> struct DctGeo {
-    static constexpr int L = N1 * N2, NL = L / 2 + 1;
-    static constexpr int MX = N1 > N2 ? N1 : N2, MH = MX / 2, H1 = N1 / 2, H2 = N2 / 2;
-    static constexpr int G = 2 * MX <= 16 ? 16 : (2 * MX <= 32 ? 32 : 64); // lanes per line
-    static constexpr int NG = 64 / G;                                       // lines per wavefront
-    static constexpr int LB0 = TwoPass<N1, N2>::LDS;
-    static constexpr int LB = LB0 > L ? LB0 : L; // complex entries per line buffer
-    static constexpr int LINES = kDctWaves * NG;  // lines in flight per block
-    static constexpr int Q = sizeof(R) == 8 ? 8 : 16; // columns per block
-    static constexpr int CPL = Q / LINES > 0 ? Q / LINES : 1; // columns per line buffer (rounds)
-    static constexpr int QB = CPL * LINES;                      // columns per block (>= Q)
-    static constexpr int NLG = (NL + G - 1) / G;                // output rows per lane and column
-    static constexpr int BUF = LINES * LB > NL * QB ? LINES * LB : NL * QB;
-};
 
 // Column q's value of field f of the backward with the curl, FP64 in the (lane, parity) layout of
 // wave.hpp, plus (f == 1) the curl part of the pressure wall condition.
@@ -118,81 +100,82 @@ __device__ inline void curl_field(const DctArgs &a, int f, int q, bool valid, zd
     }
 }
 
-// grid: (column groups x fields), XCD-ordered; block: kDctWaves wavefronts.
-template <typename R, int N1, int N2, int CH>
-__global__ void __launch_bounds__(64 * kDctWaves) k_ybwd_dct(DctArgs a) {
-    using D = DctGeo<R, N1, N2>;
-    constexpr int L = D::L, NL = D::NL, G = D::G, NG = D::NG, LB = D::LB, QB = D::QB, CPL = D::CPL;
-    constexpr int H1 = D::H1, H2 = D::H2, MH = D::MH, NLG = D::NLG;
-    __shared__ cpx<R> tw[L];
+// Each FFT row sits on one lane, two or four lines per wavefront, the root table is read from
+// global memory (cached), and pass A is pruned to the nonzero part of the line: the backward line
+// holds the NYS + 1 coefficients of L = 2 NY2 slots, the forward line the NY2 + 1 samples.  NZA is
+// the count of pass-A inputs per row that can be nonzero (n = t + N1 n2 with n2 < NZA), a
+// compile-time bound on the line's nonzero prefix.
+template <typename R, int N1, int N2> struct Dct2Geo {
+    static constexpr int L = N1 * N2, NL = L / 2 + 1;
+    static constexpr int MX = N1 > N2 ? N1 : N2;
+    static constexpr int G = MX <= 16 ? 16 : 32; // lanes per line
+    static constexpr int NG = 64 / G;            // lines per wavefront
+    static constexpr int LB0 = TwoPass<N1, N2>::LDS;
+    static constexpr int LB = LB0 > L ? LB0 : L;
+    static constexpr int W = sizeof(R) == 8 ? 4 : 8; // wavefronts per block
+    static constexpr int QB = W * NG;                // columns per block
+    static constexpr int NLG = (NL + G - 1) / G;
+    static constexpr int BUF = QB * LB > NL * QB ? QB * LB : NL * QB;
+};
+
+template <typename R, int N1, int N2, int CH, int NZA>
+__global__ void __launch_bounds__((64 * Dct2Geo<R, N1, N2>::W)) k_ybwd_dct2(DctArgs a) {
+    using D = Dct2Geo<R, N1, N2>;
+    constexpr int L = D::L, NL = D::NL, G = D::G, NG = D::NG, LB = D::LB, QB = D::QB, MX = D::MX, NLG = D::NLG;
     __shared__ cpx<R> buf[D::BUF];
+    const cpx<R> *tw = reinterpret_cast<const cpx<R> *>(a.tw);
     const int Lb = xcd_block((int)gridDim.x);
     const int f = Lb % a.nf, cg = Lb / a.nf;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G;
-    const int t = r >> 1, h = r & 1;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
     const int line = wv * NG + g;
     cpx<R> *B = buf + line * LB;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = reinterpret_cast<const cpx<R> *>(a.tw)[i];
     const int nj = a.ny + 1;
     const R half_end = (a.ny == a.NY2) ? R(0.5) : R(1); // REDFT00 end point, present when NYS = NY2
-    cpx<R> yv[CPL][NLG];
-    for (int cr = 0; cr < CPL; ++cr) {
-        const int cl = line * CPL + cr, q = cg * QB + cl;
-        const bool valid = q < a.NQ;
-        // the spectral column, FP64, in the (lane, parity) layout of its own wavefront: the
-        // wavefront's NG lines are formed one after the other (every lane takes part)
-        for (int gg = 0; gg < NG; ++gg) {
-            const int lq = (wv * NG + gg) * CPL + cr, qq = cg * QB + lq;
-            const bool vq = qq < a.NQ;
-            zd s[2][CH];
-            if (a.curl) {
-                curl_field<CH>(a, f, qq, vq, s, lane);
-            } else {
-                load_col<CH>(a.spec[f] + (size_t)(vq ? qq : 0) * a.JS, lane, a.ny, s);
-            }
-            cpx<R> *Bg = buf + (wv * NG + gg) * LB;
+    // the wavefront's NG spectral columns, FP64, one after the other with every lane taking part
+    for (int gg = 0; gg < NG; ++gg) {
+        const int qq = cg * QB + wv * NG + gg;
+        const bool vq = qq < a.NQ;
+        zd s[2][CH];
+        if (a.curl) curl_field<CH>(a, f, qq, vq, s, lane);
+        else load_col<CH>(a.spec[f] + (size_t)(vq ? qq : 0) * a.JS, lane, a.ny, s);
+        cpx<R> *Bg = buf + (wv * NG + gg) * LB;
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-                for (int c = 0; c < CH; ++c) {
-                    const int j = jof<CH>(lane, c, p);
-                    if (j < nj) {
-                        const R sc = (j == a.NY2) ? half_end : R(1);
-                        Bg[j] = vq ? cpx<R>{sc * R(s[p][c].x), sc * R(s[p][c].y)} : cpx<R>{R(0), R(0)};
-                    }
+            for (int c = 0; c < CH; ++c) {
+                const int j = jof<CH>(lane, c, p);
+                if (j < nj) {
+                    const R sc = (j == a.NY2) ? half_end : R(1);
+                    Bg[j] = vq ? cpx<R>{sc * R(s[p][c].x), sc * R(s[p][c].y)} : cpx<R>{R(0), R(0)};
                 }
-        }
-        if (cr == 0) __syncthreads(); // the root table
-        else lsync();
-        cpx<R> x[MH];
-        sfor<0, MH>([&](auto m) { x[m] = cpx<R>{R(0), R(0)}; });
-        if (t < N1)
-            sfor<0, H2>([&](auto m) {
-                const int n = t + N1 * (2 * decltype(m)::value + h);
-                const cpx<R> v = B[n < nj ? n : 0];
-                x[m] = n < nj ? v : cpx<R>{R(0), R(0)};
-            });
-        lsync();
-        two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, B, lsync);
-        if (t < N2) sfor<0, H1>([&](auto k) { B[t + N2 * (H1 * h + decltype(k)::value)] = x[k]; });
-        lsync();
+            }
+    }
+    lsync();
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto m) { x[m] = cpx<R>{R(0), R(0)}; });
+    if (t < N1)
+        sfor<0, NZA>([&](auto m) {
+            const int n = t + N1 * decltype(m)::value;
+            const cpx<R> v = B[n < nj ? n : 0];
+            x[decltype(m)::value] = n < nj ? v : cpx<R>{R(0), R(0)};
+        });
+    lsync();
+    two_pass_pruned<R, N1, N2, +1, NZA>(x, t, tw, B, lsync);
+    if (t < N2) sfor<0, N1>([&](auto k) { B[t + N2 * decltype(k)::value] = x[k]; });
+    lsync();
+    cpx<R> yv[NLG];
 #pragma unroll
-        for (int i = 0; i < NLG; ++i) {
-            const int l = r + G * i;
-            const int lc = l <= L / 2 ? l : 0;
-            const cpx<R> p = B[lc], m = B[lc == 0 ? 0 : L - lc];
-            yv[cr][i] = {R(0.5) * (p.x + m.x), R(0.5) * (p.y + m.y)};
-        }
-        lsync();
+    for (int i = 0; i < NLG; ++i) {
+        const int l = t + G * i;
+        const int lc = l <= L / 2 ? l : 0;
+        const cpx<R> pv = B[lc], mv = B[lc == 0 ? 0 : L - lc];
+        yv[i] = {R(0.5) * (pv.x + mv.x), R(0.5) * (pv.y + mv.y)};
     }
     __syncthreads(); // every line is done with its buffer: the tile may overwrite them
-    for (int cr = 0; cr < CPL; ++cr) {
-        const int cl = line * CPL + cr;
 #pragma unroll
-        for (int i = 0; i < NLG; ++i) {
-            const int l = r + G * i;
-            if (l < NL) buf[l * QB + cl] = yv[cr][i];
-        }
+    for (int i = 0; i < NLG; ++i) {
+        const int l = t + G * i;
+        if (l < NL) buf[l * QB + line] = yv[i];
     }
     __syncthreads();
     const int q0 = cg * QB, nq = min(QB, a.NQ - q0);
@@ -203,21 +186,17 @@ __global__ void __launch_bounds__(64 * kDctWaves) k_ybwd_dct(DctArgs a) {
     }
 }
 
-// grid: (column groups x fields), XCD-ordered.
-template <typename R, int N1, int N2>
-__global__ void __launch_bounds__(64 * kDctWaves) k_yfwd_dct(DctArgs a) {
-    using D = DctGeo<R, N1, N2>;
-    constexpr int L = D::L, NL = D::NL, G = D::G, NG = D::NG, LB = D::LB, QB = D::QB, CPL = D::CPL;
-    constexpr int H1 = D::H1, H2 = D::H2, MH = D::MH;
-    __shared__ cpx<R> tw[L];
+template <typename R, int N1, int N2, int NZA>
+__global__ void __launch_bounds__((64 * Dct2Geo<R, N1, N2>::W)) k_yfwd_dct2(DctArgs a) {
+    using D = Dct2Geo<R, N1, N2>;
+    constexpr int L = D::L, NL = D::NL, G = D::G, NG = D::NG, LB = D::LB, QB = D::QB, MX = D::MX;
     __shared__ cpx<R> buf[D::BUF];
+    const cpx<R> *tw = reinterpret_cast<const cpx<R> *>(a.tw);
     const int Lb = xcd_block((int)gridDim.x);
     const int f = Lb % a.nf, cg = Lb / a.nf;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G;
-    const int t = r >> 1, h = r & 1;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, t = lane - g * G;
     const int line = wv * NG + g;
     const int q0 = cg * QB, nq = min(QB, a.NQ - q0);
-    for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = reinterpret_cast<const cpx<R> *>(a.tw)[i];
     {
         const cpx<R> *Y = reinterpret_cast<const cpx<R> *>(a.y) + (size_t)f * a.YS + q0;
         for (int e = threadIdx.x; e < NL * QB; e += blockDim.x) {
@@ -226,59 +205,66 @@ __global__ void __launch_bounds__(64 * kDctWaves) k_yfwd_dct(DctArgs a) {
         }
     }
     __syncthreads();
-    // pass-A inputs of every column of this line, weighted (1/2 at both end points)
-    cpx<R> xin[CPL][MH];
-    for (int cr = 0; cr < CPL; ++cr) {
-        const int cl = line * CPL + cr;
-        sfor<0, MH>([&](auto m) { xin[cr][m] = cpx<R>{R(0), R(0)}; });
-        if (t < N1)
-            sfor<0, H2>([&](auto m) {
-                const int n = t + N1 * (2 * decltype(m)::value + h);
-                const cpx<R> v = buf[(n < NL ? n : 0) * QB + cl];
-                const R wgt = n < NL ? ((n == 0 || n == L / 2) ? R(0.5) : R(1)) : R(0);
-                xin[cr][decltype(m)::value] = {wgt * v.x, wgt * v.y};
-            });
-    }
+    // pass-A inputs of this line, weighted (1/2 at both end points)
+    cpx<R> x[MX];
+    sfor<0, MX>([&](auto m) { x[m] = cpx<R>{R(0), R(0)}; });
+    if (t < N1)
+        sfor<0, NZA>([&](auto m) {
+            const int n = t + N1 * decltype(m)::value;
+            const cpx<R> v = buf[(n < NL ? n : 0) * QB + line];
+            const R wgt = n < NL ? ((n == 0 || n == L / 2) ? R(0.5) : R(1)) : R(0);
+            x[decltype(m)::value] = {wgt * v.x, wgt * v.y};
+        });
     __syncthreads(); // the tile is read: the lines may overwrite it
     cpx<R> *B = buf + line * LB;
-    const int nj = a.ny + 1;
-    const double inv = 1.0 / (double)(L / 2);
-    for (int cr = 0; cr < CPL; ++cr) {
-        const int q = q0 + line * CPL + cr;
-        const bool valid = q < a.NQ;
-        cpx<R> x[MH];
-        sfor<0, MH>([&](auto m) { x[m] = xin[cr][m]; });
-        two_pass_pair<R, N1, N2, -1, false>(x, t, h, tw, B, lsync);
-        if (t < N2) sfor<0, H1>([&](auto k) { B[t + N2 * (H1 * h + decltype(k)::value)] = x[k]; });
-        lsync();
-        if (valid) {
-            zd *out = a.spec_out[f] + (size_t)q * a.JS;
-            const zd *add = a.add[f] ? a.add[f] + (size_t)q * a.JS : nullptr;
-            for (int j = r; j < a.JS; j += G) {
-                zd v = {0.0, 0.0};
-                if (j < nj) {
-                    const cpx<R> p = B[j], m = B[j == 0 ? 0 : L - j];
-                    const double sc = j == 0 ? 0.5 * inv : inv;
-                    v = {sc * ((double)p.x + (double)m.x), sc * ((double)p.y + (double)m.y)};
-                    if (add) v = v + add[j];
-                }
-                out[j] = v;
+    two_pass_pruned<R, N1, N2, -1, NZA>(x, t, tw, B, lsync);
+    if (t < N2) sfor<0, N1>([&](auto k) { B[t + N2 * decltype(k)::value] = x[k]; });
+    lsync();
+    const int q = q0 + line;
+    if (q < a.NQ) {
+        const int nj = a.ny + 1;
+        const double inv = 1.0 / (double)(L / 2);
+        zd *out = a.spec_out[f] + (size_t)q * a.JS;
+        const zd *add = a.add[f] ? a.add[f] + (size_t)q * a.JS : nullptr;
+        for (int j = t; j < a.JS; j += G) {
+            zd v = {0.0, 0.0};
+            if (j < nj) {
+                const cpx<R> pv = B[j], mv = B[j == 0 ? 0 : L - j];
+                const double sc = j == 0 ? 0.5 * inv : inv;
+                v = {sc * ((double)pv.x + (double)mv.x), sc * ((double)pv.y + (double)mv.y)};
+                if (add) v = v + add[j];
             }
+            out[j] = v;
         }
-        lsync();
     }
 }
 
-template <typename R, int N1, int N2> static void go_ybwd_dct(const DctArgs &a, hipStream_t st) {
-    using D = DctGeo<R, N1, N2>;
+// pass-A pruning bound: lines whose nonzero prefix fits NZ1 = ceil((L/3 + 2) / N1) entries per row
+// (dealiased meshes, NYS <= 2 NY2 / 3 + 1) take the pruned kernels, others the full ones
+template <int L, int N1> constexpr int nza_small() { return (L / 3 + 2 + N1 - 1) / N1; }
+
+template <typename R, int N1, int N2> static bool go_ybwd_dct2(const DctArgs &a, hipStream_t st) {
+    using D = Dct2Geo<R, N1, N2>;
+    constexpr int L = N1 * N2, NZS = nza_small<L, N1>() < N2 ? nza_small<L, N1>() : N2;
+    constexpr int NZF = (L / 2 + 1 + N1 - 1) / N1 < N2 ? (L / 2 + 1 + N1 - 1) / N1 : N2;
     const int ncg = (a.NQ + D::QB - 1) / D::QB;
-    if (a.CH == 1) hipLaunchKernelGGL((k_ybwd_dct<R, N1, N2, 1>), dim3(ncg * a.nf), dim3(64 * kDctWaves), 0, st, a);
-    else hipLaunchKernelGGL((k_ybwd_dct<R, N1, N2, 2>), dim3(ncg * a.nf), dim3(64 * kDctWaves), 0, st, a);
+    const dim3 grid(ncg * a.nf), block(64 * D::W);
+    const bool small = a.ny + 1 <= N1 * NZS;
+    if (a.CH == 1) {
+        if (small) hipLaunchKernelGGL((k_ybwd_dct2<R, N1, N2, 1, NZS>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_ybwd_dct2<R, N1, N2, 1, NZF>), grid, block, 0, st, a);
+    } else {
+        if (small) hipLaunchKernelGGL((k_ybwd_dct2<R, N1, N2, 2, NZS>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_ybwd_dct2<R, N1, N2, 2, NZF>), grid, block, 0, st, a);
+    }
+    return true;
 }
-template <typename R, int N1, int N2> static void go_yfwd_dct(const DctArgs &a, hipStream_t st) {
-    using D = DctGeo<R, N1, N2>;
+template <typename R, int N1, int N2> static bool go_yfwd_dct2(const DctArgs &a, hipStream_t st) {
+    using D = Dct2Geo<R, N1, N2>;
+    constexpr int L = N1 * N2, NZF = (L / 2 + 1 + N1 - 1) / N1 < N2 ? (L / 2 + 1 + N1 - 1) / N1 : N2;
     const int ncg = (a.NQ + D::QB - 1) / D::QB;
-    hipLaunchKernelGGL((k_yfwd_dct<R, N1, N2>), dim3(ncg * a.nf), dim3(64 * kDctWaves), 0, st, a);
+    hipLaunchKernelGGL((k_yfwd_dct2<R, N1, N2, NZF>), dim3(ncg * a.nf), dim3(64 * D::W), 0, st, a);
+    return true;
 }
 
 // Line lengths L = 2 NY2 with an instantiation (even factors up to 24).
@@ -293,12 +279,12 @@ bool dct_supported(int L) {
 }
 
 bool launch_ybwd_dct(const DctArgs &a, bool fp32, hipStream_t st) {
-#define X(NN, A, B) case NN: if (fp32) go_ybwd_dct<float, A, B>(a, st); else go_ybwd_dct<double, A, B>(a, st); return true;
+#define X(NN, A, B) case NN: return fp32 ? go_ybwd_dct2<float, A, B>(a, st) : go_ybwd_dct2<double, A, B>(a, st);
     switch (a.L) { CDNS_DCT_SIZES(X) default: return false; }
 #undef X
 }
 bool launch_yfwd_dct(const DctArgs &a, bool fp32, hipStream_t st) {
-#define X(NN, A, B) case NN: if (fp32) go_yfwd_dct<float, A, B>(a, st); else go_yfwd_dct<double, A, B>(a, st); return true;
+#define X(NN, A, B) case NN: return fp32 ? go_yfwd_dct2<float, A, B>(a, st) : go_yfwd_dct2<double, A, B>(a, st);
     switch (a.L) { CDNS_DCT_SIZES(X) default: return false; }
 #undef X
 }
