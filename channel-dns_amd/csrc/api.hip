@@ -142,6 +142,7 @@ struct cdns_ctx {
     cdns_config cfg{};
     Geometry g{};
     bool fp32 = false;
+    bool nl32 = false; // the step's nonlinear term and its history in single precision (mixed)
     hipStream_t st = nullptr;
     std::string err;
     size_t esz = 16; // bytes per physical complex element
@@ -384,7 +385,9 @@ void backward_yz(cdns_ctx *c, const zd *const *planes, int n, void *zout = nullp
     backward_z(c, n, zout);
 }
 // ZN fields (n, the context's ZN buffer or zin) -> Y fields -> spectral planes (+ add planes)
-void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr, zd *const *add = nullptr) {
+// (out32: the planes hold single-precision complex values)
+void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr, zd *const *add = nullptr,
+                bool out32 = false) {
     {
         Launch L(c, "z_forward");
         launch_zfwd(zargs(c, zin ? zin : c->ZNbuf, c->Ybuf, c->g.ZS, c->g.YS), n, c->fp32, c->st);
@@ -396,6 +399,7 @@ void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr
             a.add[f] = add ? add[f] : nullptr;
         }
         a.nf = n;
+        a.out32 = out32;
         Launch L(c, "y_forward");
         if (!launch_yfwd_dct(a, c->fp32, c->st)) throw std::logic_error("no DCT y stage for this NY2");
         return;
@@ -403,6 +407,7 @@ void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr
     YArgs a = yargs(c);
     a.coef = c->ycoef_fe;
     a.coef2 = c->ycoef_fo;
+    a.out32 = out32;
     for (int f = 0; f < n; ++f) {
         a.spec_out[f] = planes[f];
         a.add[f] = add ? add[f] : nullptr;
@@ -414,14 +419,15 @@ void forward_zy(cdns_ctx *c, zd *const *planes, int n, const void *zin = nullptr
 }
 
 // N = a x b (+ add) for spectral vector planes (a: 3 planes, b: 3 planes)
-void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out, zd *const *add = nullptr) {
+void cross_planes(cdns_ctx *c, zd *const *a, zd *const *b, zd *const *out, zd *const *add = nullptr,
+                  bool out32 = false) {
     const zd *f6[6] = {a[0], a[1], a[2], b[0], b[1], b[2]};
     backward_yz(c, f6, 6);
     {
         Launch L(c, "x_product");
         launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
     }
-    forward_zy(c, out, 3, nullptr, add);
+    forward_zy(c, out, 3, nullptr, add, out32);
 }
 
 // ------------------------------------------------------------- coarse grid ----
@@ -608,7 +614,7 @@ void curl_planes(cdns_ctx *c, zd *const *u, zd *const *w, zd *bcw) {
 }
 
 // TimeStepper::enqueue_nonlinear_term on the context's velocity planes
-void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
+void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl, bool out32 = false) {
     zd *const *add = c->body_on ? c->body : nullptr;
     if (c->ydct && c->ycurl) {
         // DCT-I y backward of u with the curl formed per column: the six Y fields and the wall
@@ -627,7 +633,7 @@ void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
             Launch L(c, "x_product");
             launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
         }
-        forward_zy(c, nl, 3, nullptr, add);
+        forward_zy(c, nl, 3, nullptr, add, out32);
         return;
     }
     if (c->ymfma && c->ycurl) {
@@ -651,12 +657,12 @@ void nonlinear_planes(cdns_ctx *c, zd *const *u, zd *const *nl) {
                 Launch L(c, "x_product");
                 launch_xprod(xargs(c, c->Zbuf, c->ZNbuf, c->g.ZS, c->g.ZS, c->P), c->fp32, c->st);
             }
-            forward_zy(c, nl, 3, nullptr, add);
+            forward_zy(c, nl, 3, nullptr, add, out32);
             return;
         }
     }
     curl_planes(c, u, c->om, c->bcw);
-    cross_planes(c, u, c->om, nl, add);
+    cross_planes(c, u, c->om, nl, add, out32);
 }
 
 // ---------------------------------------------------------------- solver --
@@ -795,8 +801,9 @@ void step(cdns_ctx *c, int it) {
         if (c->g_v != co[4]) setup(c, co[4]);
     }
     c->time += c->cfg.DT;
-    nonlinear_planes(c, c->u, c->nl);
+    nonlinear_planes(c, c->u, c->nl, c->nl32);
     SolveArgs a = solve_args(c, MODE_STEP);
+    a.nl32 = c->nl32;
     for (int k = 0; k < 3; ++k) {
         a.u[k] = c->u[k];
         a.up[k] = c->up[k];
@@ -1209,6 +1216,10 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         if (cfg->precision != CDNS_FP64 && cfg->precision != CDNS_MIXED)
             throw std::invalid_argument("precision must be CDNS_FP64 or CDNS_MIXED");
         c->fp32 = cfg->precision == CDNS_MIXED;
+        // the mixed step keeps its nonlinear term (the FP32 transforms' output) and that term's
+        // history in single precision; CDNS_MIXED_N=f64 keeps them in FP64
+        c->nl32 = c->fp32;
+        if (const char *e = getenv("CDNS_MIXED_N")) c->nl32 = c->fp32 && std::string(e) != "f64";
         if (const char *e = getenv("CDNS_FFT")) c->rf = std::string(e) != "stockham";
         if (const char *e = getenv("CDNS_YSTAGE")) {
             c->ymfma = std::string(e) != "vector";
@@ -1340,8 +1351,13 @@ int cdns_create(const cdns_config *cfg, cdns_ctx **out) {
         for (int k = 0; k < 3; ++k) {
             C->u[k] = C->alloc<zd>(g.S);
             C->up[k] = C->alloc<zd>(g.S);
-            C->nl[k] = C->alloc<zd>(g.S);
-            C->nlp[k] = C->alloc<zd>(g.S);
+            if (C->nl32) {
+                C->nl[k] = reinterpret_cast<zd *>(C->alloc<cpx<float>>(g.S));
+                C->nlp[k] = reinterpret_cast<zd *>(C->alloc<cpx<float>>(g.S));
+            } else {
+                C->nl[k] = C->alloc<zd>(g.S);
+                C->nlp[k] = C->alloc<zd>(g.S);
+            }
             C->om[k] = C->alloc<zd>(g.S);
         }
         C->P_ = C->alloc<zd>(g.S);
@@ -1752,7 +1768,14 @@ int cdns_set_state(cdns_ctx *c, int field, const double *data, int where) {
         zd *pl[3] = {};
         int ncomp, nrow, JS;
         state_planes(c, field, pl, ncomp, nrow, JS);
-        ref_to_planes(c, data, ncomp, ncomp, nrow, pl, where, JS);
+        if (c->nl32 && (field == 2 || field == 3)) {
+            // single-precision planes: staged through FP64 planes
+            zd *st[3] = {c->scr[0], c->scr[1], c->scr[2]};
+            ref_to_planes(c, data, ncomp, ncomp, nrow, st, where, JS);
+            for (int k = 0; k < 3; ++k) launch_narrow(st[k], pl[k], c->g.S, c->st);
+        } else {
+            ref_to_planes(c, data, ncomp, ncomp, nrow, pl, where, JS);
+        }
         if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));
         return 0;
     } catch (const std::exception &e) {
@@ -1765,6 +1788,10 @@ int cdns_get_state(cdns_ctx *c, int field, double *data, int where) {
         zd *pl[3] = {};
         int ncomp, nrow, JS;
         state_planes(c, field, pl, ncomp, nrow, JS);
+        if (c->nl32 && (field == 2 || field == 3)) {
+            for (int k = 0; k < 3; ++k) launch_widen(pl[k], c->scr[k], c->g.S, c->st);
+            for (int k = 0; k < 3; ++k) pl[k] = c->scr[k];
+        }
         const zd *cp[3] = {pl[0], pl[1], pl[2]};
         planes_to_ref(c, cp, ncomp, nrow, data, ncomp, where, JS);
         if (where == CDNS_HOST) CDNS_HIP(hipStreamSynchronize(c->st));

@@ -24,6 +24,7 @@ struct YArgs {
     double Re;
     int nzk;
     zd *bcw;
+    int out32; // forward: spec_out holds single-precision complex planes (the mixed step's N)
 };
 
 // y stages as a DCT-I through complex FFTs of length L = 2 NY2 (transforms_dct.hip).
@@ -38,6 +39,7 @@ struct DctArgs {
     double Re;
     zd *bcw;           // curl: the curl part of the pressure wall condition [q][2], or null
     int ny, JS, NQ, nzk, NY2, L, nf, curl, CH;
+    int out32;         // forward: spec_out holds single-precision complex planes
 };
 bool dct_supported(int L);
 bool launch_ybwd_dct(const DctArgs &a, bool fp32, hipStream_t st);
@@ -114,6 +116,9 @@ struct LayoutArgs {
 void launch_ref_to_planes(const LayoutArgs &a, hipStream_t st);
 void launch_planes_to_ref(const LayoutArgs &a, hipStream_t st);
 void launch_real_planes_to_ref(const double *plane, double *ref, int nrow, int nkx, int nzk, int JS, hipStream_t st);
+// n complex entries: single-precision plane -> FP64 plane, and back
+void launch_widen(const void *src, zd *dst, size_t n, hipStream_t st);
+void launch_narrow(const zd *src, void *dst, size_t n, hipStream_t st);
 
 struct CurlArgs {
     const zd *u[3];
@@ -138,6 +143,7 @@ struct SolveArgs {
     double aV[3], bV[3];
     // rhs sources
     const zd *u[3], *up[3], *nl[3], *nlp[3]; // step: a0 u + a1 up + b0 nl + b1 nlp
+    int nl32;                                // step: nl, nlp hold single-precision complex planes
     double a0, a1, b0, b1;
     const zd *rv[3]; // given velocity rhs planes
     const zd *rp;    // given pressure rhs plane (null: fused divergence)

@@ -125,7 +125,7 @@ __device__ void tau_solve(const zd (&r)[NR][NE], zd (&out)[NR][NE], const double
 // (timestepper.cpp:26-43), correct_zero_mode (timestepper.cpp:60-68) and correct_flow
 // (simulation.cpp:111-140) fused in MODE_STEP; the setup problems of eqsolve_pv.cpp:266-303
 // and 452-477 in MODE_INFLUENCE / MODE_BPROBLEM.
-template <int NE, int MODE>
+template <int NE, int MODE, bool NL32 = false>
 __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
     const int lane = lane_id(), p = sp::parity(lane);
     // the column index is wavefront-uniform: made visible as such, the per-column scalars
@@ -154,8 +154,13 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
             zd u[NE], up[NE], n[NE], np[NE];
             sp::load_col<NE>(a.u[k] + off, lane, ny, u);
             sp::load_col<NE>(a.up[k] + off, lane, ny, up);
-            sp::load_col<NE>(a.nl[k] + off, lane, ny, n);
-            sp::load_col<NE>(a.nlp[k] + off, lane, ny, np);
+            if constexpr (NL32) {
+                sp::load_col<NE>(reinterpret_cast<const cpx<float> *>(a.nl[k]) + off, lane, ny, n);
+                sp::load_col<NE>(reinterpret_cast<const cpx<float> *>(a.nlp[k]) + off, lane, ny, np);
+            } else {
+                sp::load_col<NE>(a.nl[k] + off, lane, ny, n);
+                sp::load_col<NE>(a.nlp[k] + off, lane, ny, np);
+            }
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
                 rv[k][e] = ((a.a0 * u[e] + a.a1 * up[e]) + a.b0 * n[e]) + a.b1 * np[e];
@@ -445,7 +450,10 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
 template <int NE> static void launch_solve_mode(const SolveArgs &a, hipStream_t st) {
     dim3 grid((a.NQ + 3) / 4), block(256);
     switch (a.mode) {
-    case MODE_STEP: hipLaunchKernelGGL((k_solve<NE, MODE_STEP>), grid, block, 0, st, a); break;
+    case MODE_STEP:
+        if (a.nl32) hipLaunchKernelGGL((k_solve<NE, MODE_STEP, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_solve<NE, MODE_STEP>), grid, block, 0, st, a);
+        break;
     case MODE_GIVEN: hipLaunchKernelGGL((k_solve<NE, MODE_GIVEN>), grid, block, 0, st, a); break;
     case MODE_INFLUENCE: hipLaunchKernelGGL((k_solve<NE, MODE_INFLUENCE>), grid, block, 0, st, a); break;
     case MODE_BPROBLEM: hipLaunchKernelGGL((k_solve<NE, MODE_BPROBLEM>), grid, block, 0, st, a); break;

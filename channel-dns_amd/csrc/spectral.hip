@@ -128,6 +128,22 @@ static dim3 grid_for(size_t n) {
     if (b > 65535 * 4) b = 65535 * 4;
     return dim3((unsigned)(b ? b : 1));
 }
+
+// n complex entries between an FP64 plane and a single-precision plane (the mixed step's N)
+__global__ void k_widen(const cpx<float> *src, zd *dst, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = {double(src[e].x), double(src[e].y)};
+}
+__global__ void k_narrow(const zd *src, cpx<float> *dst, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = {float(src[e].x), float(src[e].y)};
+}
+void launch_widen(const void *src, zd *dst, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_widen, grid_for(n), dim3(256), 0, st, reinterpret_cast<const cpx<float> *>(src), dst, n);
+}
+void launch_narrow(const zd *src, void *dst, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_narrow, grid_for(n), dim3(256), 0, st, src, reinterpret_cast<cpx<float> *>(dst), n);
+}
 void launch_ref_to_planes(const LayoutArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(k_ref_to_planes, grid_for((size_t)a.nkx * a.nzk * a.JS), dim3(256), 0, st, a);
 }

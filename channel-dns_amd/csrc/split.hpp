@@ -253,6 +253,15 @@ template <int NE> __device__ inline void load_col(const zd *__restrict__ col, in
         v[e] = j <= ny ? t : zd{0.0, 0.0};
     }
 }
+// The same from a single-precision column, widened.
+template <int NE> __device__ inline void load_col(const cpx<float> *__restrict__ col, int lane, int ny, zd (&v)[NE]) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const int j = jof<NE>(lane, e);
+        const cpx<float> t = col[j <= ny ? j : ny];
+        v[e] = j <= ny ? zd{double(t.x), double(t.y)} : zd{0.0, 0.0};
+    }
+}
 template <int NE> __device__ inline void load_rcol(const double *__restrict__ col, int lane, int ny, double (&v)[NE]) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {

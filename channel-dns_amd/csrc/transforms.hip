@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(512) k_yfwd(YArgs a) {
     }
     if (j >= a.nj) return;
     zd *out = a.spec_out[f];
+    cpx<float> *out32 = reinterpret_cast<cpx<float> *>(a.spec_out[f]);
     const zd *add = a.add[f];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -158,7 +159,8 @@ __global__ void __launch_bounds__(512) k_yfwd(YArgs a) {
         if (q < a.NQ) {
             zd v = {double(acc[c].x), double(acc[c].y)};
             if (add) v = v + add[(size_t)q * a.JS + j];
-            out[(size_t)q * a.JS + j] = v;
+            if (a.out32) out32[(size_t)q * a.JS + j] = {float(v.x), float(v.y)};
+            else out[(size_t)q * a.JS + j] = v;
         }
     }
 }

@@ -225,6 +225,7 @@ __global__ void __launch_bounds__((64 * Dct2Geo<R, N1, N2>::W)) k_yfwd_dct2(DctA
         const int nj = a.ny + 1;
         const double inv = 1.0 / (double)(L / 2);
         zd *out = a.spec_out[f] + (size_t)q * a.JS;
+        cpx<float> *out32 = reinterpret_cast<cpx<float> *>(a.spec_out[f]) + (size_t)q * a.JS;
         const zd *add = a.add[f] ? a.add[f] + (size_t)q * a.JS : nullptr;
         for (int j = t; j < a.JS; j += G) {
             zd v = {0.0, 0.0};
@@ -234,7 +235,8 @@ __global__ void __launch_bounds__((64 * Dct2Geo<R, N1, N2>::W)) k_yfwd_dct2(DctA
                 v = {sc * ((double)pv.x + (double)mv.x), sc * ((double)pv.y + (double)mv.y)};
                 if (add) v = v + add[j];
             }
-            out[j] = v;
+            if (a.out32) out32[j] = {float(v.x), float(v.y)};
+            else out[j] = v;
         }
     }
 }

@@ -658,22 +658,22 @@ __global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
                 ym = {pm[0], pm[1]};
             }
             zd *out = a.spec_out[f] + (size_t)q * a.JS;
+            cpx<float> *out32 = reinterpret_cast<cpx<float> *>(a.spec_out[f]) + (size_t)q * a.JS;
             const zd *add = a.add[f] ? a.add[f] + (size_t)q * a.JS : nullptr;
+            auto put = [&](int j, zd v) {
+                if (add) v = v + add[j];
+                if (a.out32) out32[j] = {float(v.x), float(v.y)};
+                else out[j] = v;
+            };
 #pragma unroll
             for (int t = 0; t < JTP; ++t) {
                 const int m = m0 + 16 * t + col;
                 const R fm = Fm[16 * t + col];
                 if (m < nje) {
                     const R re = accE[c][t][2 * h] + fm * ym.x, im = accE[c][t][2 * h + 1] + fm * ym.y;
-                    zd v = {double(re), double(im)};
-                    if (add) v = v + add[2 * m];
-                    out[2 * m] = v;
+                    put(2 * m, zd{double(re), double(im)});
                 }
-                if (m < njo) {
-                    zd v = {double(accO[c][t][2 * h]), double(accO[c][t][2 * h + 1])};
-                    if (add) v = v + add[2 * m + 1];
-                    out[2 * m + 1] = v;
-                }
+                if (m < njo) put(2 * m + 1, zd{double(accO[c][t][2 * h]), double(accO[c][t][2 * h + 1])});
             }
         }
     }

@@ -99,7 +99,9 @@ int cdns_get_table(cdns_ctx *ctx, int which, double *out);
 
 /* Simulation state (src/sim/simulation.hpp:44-55).  field: 0 velocity, 1 velocity_prev,
  * 2 nonlinear, 3 nonlinear_prev, 4 pressure, 5 pressure_bc, 6 pressure_bc_prev, 7 velocity_bc.
- * The multistep history (1, 3, 6) is what a restart file does not hold. */
+ * The multistep history (1, 3, 6) is what a restart file does not hold.  A CDNS_MIXED context
+ * keeps fields 2 and 3 (the step's nonlinear term and its history) in single precision: values
+ * set there are rounded to float, values read are widened. */
 int cdns_set_state(cdns_ctx *ctx, int field, const double *data, int where);
 int cdns_get_state(cdns_ctx *ctx, int field, double *data, int where);
 /* One iteration `it` (>= 1) of Simulation::run's serial body (src/sim/simulation.cpp:173-191):

@@ -59,3 +59,36 @@ def test_mixed_channel32_100_steps(gpu):
     cfg = config.channel_32()
     u0 = fields.seeded_velocity(cfg, seed=5, mean="log", spectrum="k53", amplitude=0.1, kcut=8)
     _check(gpu, "channel_32", cfg, u0, 100)
+
+
+def test_mixed_nonlinear_term_single_precision_storage(gpu):
+    """The mixed step keeps its nonlinear term and that term's history as single-precision
+    planes: state written to them comes back rounded to FP32 (and widened exactly); an FP64
+    context keeps them exact."""
+    cfg = config.channel_32()
+    rng = np.random.default_rng(11)
+    for precision, exact in ((gpu.MIXED, False), (gpu.FP64, True)):
+        dev = gpu.Channel(cfg, precision=precision)
+        for name in ("nonlinear", "nonlinear_prev"):
+            shape = dev.get_state(name).shape
+            x = rng.standard_normal(shape)
+            dev.set_state(name, x)
+            back = dev.get_state(name)
+            want = x if exact else x.astype(np.float32).astype(np.float64)
+            assert np.array_equal(back, want), (precision, name)
+        dev.close()
+
+
+def test_mixed_single_precision_nonlinear_term_is_lossless(gpu, monkeypatch):
+    """The matrix-core y forward of the mixed step forms N in FP32, so single-precision storage
+    of N and its history loses nothing: 100 mixed steps with N kept in FP32 planes and with N kept
+    in FP64 planes (CDNS_MIXED_N=f64) give the same velocity, bit for bit (no body force)."""
+    cfg = config.retau180(dt=5e-5)
+    u0 = fields.seeded_velocity(cfg, seed=180, mean="log", spectrum="k53", amplitude=0.1)
+    monkeypatch.delenv("CDNS_YSTAGE", raising=False)
+    monkeypatch.delenv("CDNS_MIXED_N", raising=False)
+    got32 = _run(gpu, cfg, u0, 100, gpu.MIXED)
+    monkeypatch.setenv("CDNS_MIXED_N", "f64")
+    got64 = _run(gpu, cfg, u0, 100, gpu.MIXED)
+    assert np.isfinite(got32).all()
+    assert np.array_equal(got32, got64)
