@@ -74,11 +74,10 @@ __device__ inline void fill_lds(R *dst, const double *src, int n, Idx idx) {
         }
     }
 }
-// fused backward: row tiles per part (five accumulator sets each), waves per block, steps the
-// loads run ahead, LDS budget of its two tables
-constexpr int kYCurlTiles = 2;
-constexpr int kYCurlWaves64 = 4, kYCurlWaves32 = 8;
-constexpr int kYCurlDepth = 1;
+// fused backward: one row tile (16 l') per wavefront and 4-wavefront blocks in both precisions
+// (FP32: 112 VGPRs, 4 waves per SIMD; 50.9 against 53.6 us with two tiles on 8-wavefront
+// blocks at Re_tau=180); the loads are double-buffered; LDS budget of its two tables
+constexpr int kYCurlWaves = 4;
 constexpr int kYCurlLds = 64 * 1024;
 // forward: waves per block (the parts' tables are shared by all of a block's column groups)
 constexpr int kYFwdWaves = 8;
@@ -290,14 +289,13 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
     const int col = lane & 15, k = lane >> 4, hh = col & 1;
     const int q = (cg * W + wv) * 8 + (col >> 1);
     const bool okq = q < a.NQ;
-    const size_t qoff = (size_t)(okq ? q : 0) * a.JS;
-    const zd *src[3];
-    const double *srcp[3];
+    // byte offset of this lane's first value in every velocity plane (the planes share a layout):
+    // loads are a uniform plane base plus this 32-bit offset plus a constant
+    const unsigned colb = (unsigned)((okq ? q : 0) * a.JS) * 16u;
+    const char *pl[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        src[f] = a.spec[f] + qoff;
-        srcp[f] = reinterpret_cast<const double *>(a.spec[f]) + qoff * 2 + hh;
-    }
+    for (int f = 0; f < 3; ++f) pl[f] = reinterpret_cast<const char *>(a.spec[f]);
+    const unsigned lane_off = kFused ? colb + (unsigned)(2 * k + hh) * 16u : colb + (unsigned)(2 * k) * 16u + (unsigned)hh * 8u;
     // accumulators: 0 U, 1 V, 2 W (T), 3 U', 4 W' (T')
     v4 accE[5][TRP], accO[5][TRP];
     R mid[5];
@@ -317,37 +315,44 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
     const int nsteps = a.njp / 8;
     // two load buffers, so that the loads of step s + 1 land in registers that step s does not
     // read (no copies that would wait for them)
-    struct Buf { // raw loads; j >= nj is masked where they are used
+    struct Buf { // raw loads; j >= nj (inside the column's padding, j < njp <= JS) masked in the last step
         zd v[3];
         double e[3], o[3];
     };
     auto load = [&](Buf &b, int s) {
-        const int je = 8 * s + 2 * k;
+        // steps past the last (prefetch) reload the last step, inside the column
+        const unsigned o = lane_off + (unsigned)(s < nsteps ? s : nsteps - 1) * 128u;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             if constexpr (kFused) {
-                const int j = je + hh;
-                b.v[f] = src[f][j < nj ? j : 0];
+                b.v[f] = *reinterpret_cast<const zd *>(pl[f] + o);
             } else {
-                const int jo = je + 1;
-                b.e[f] = srcp[f][2 * (je < nj ? je : 0)];
-                b.o[f] = srcp[f][2 * (jo < nj ? jo : 0)];
+                b.e[f] = *reinterpret_cast<const double *>(pl[f] + o);
+                b.o[f] = *reinterpret_cast<const double *>(pl[f] + o + 16u);
             }
         }
     };
     auto compute = [&](const Buf &b, int s) {
         const int je = 8 * s + 2 * k;
-        const double mkE = je < nj ? 1.0 : 0.0, mkO = je + 1 < nj ? 1.0 : 0.0;
         zd v[3];
         double e[3], o[3];
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            if constexpr (kFused) {
-                const double mk = hh ? mkO : mkE;
-                v[f] = {b.v[f].x * mk, b.v[f].y * mk};
-            } else {
-                e[f] = b.e[f] * mkE;
-                o[f] = b.o[f] * mkO;
+            v[f] = b.v[f];
+            e[f] = b.e[f];
+            o[f] = b.o[f];
+        }
+        if (s == nsteps - 1) { // the only step that can reach j >= nj
+            const double mkE = je < nj ? 1.0 : 0.0, mkO = je + 1 < nj ? 1.0 : 0.0;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                if constexpr (kFused) {
+                    const double mk = hh ? mkO : mkE;
+                    v[f] = {v[f].x * mk, v[f].y * mk};
+                } else {
+                    e[f] *= mkE;
+                    o[f] *= mkO;
+                }
             }
         }
         if (do_wall) {
@@ -407,30 +412,15 @@ __global__ void __launch_bounds__(64 * W) k_ybwd_curl(YArgs a) {
             mid[4] += md * cO[2];
         }
     };
-    // loads past the last step read clamped (in-plane) addresses and are not used: no branch
-    // around a load, so the compiler counts the loads in flight exactly
-    if constexpr (kYCurlDepth == 2) {
-        // three buffers: the loads run two steps ahead
-        Buf b0, b1, b2;
-        load(b0, 0);
-        load(b1, 1);
-        for (int s = 0; s < nsteps; s += 3) {
-            load(b2, s + 2);
-            compute(b0, s);
-            load(b0, s + 3);
-            if (s + 1 < nsteps) compute(b1, s + 1);
-            load(b1, s + 4);
-            if (s + 2 < nsteps) compute(b2, s + 2);
-        }
-    } else {
-        Buf b0, b1;
-        load(b0, 0);
-        for (int s = 0; s < nsteps; s += 2) {
-            load(b1, s + 1);
-            compute(b0, s);
-            load(b0, s + 2);
-            if (s + 1 < nsteps) compute(b1, s + 1);
-        }
+    // loads past the last step reread the last step (inside the column) and are not used: no
+    // branch around a load, so the compiler counts the loads in flight exactly
+    Buf b0, b1;
+    load(b0, 0);
+    for (int s = 0; s < nsteps; s += 2) {
+        load(b1, s + 1);
+        compute(b0, s);
+        load(b0, s + 2);
+        if (s + 1 < nsteps) compute(b1, s + 1);
     }
 
     const int kxi = okq ? q / a.nzk : 0, kzi = okq ? q - kxi * a.nzk : 0;
@@ -757,19 +747,10 @@ template <typename R, int TRP, int W> static void go_ybwd_curl(YArgs a, int part
 bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
     y_shape(a);
     const size_t esz = fp32 ? 4 : 8;
-    if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * 16 * esz + 2 * a.njp * esz > (size_t)kYCurlLds) return false;
-    const int TR = (a.rows_m + 15) / 16;
-    int fit = std::min(kYCurlTiles, TR);
-    while (fit > 1 && (size_t)2 * a.njp * esz * (16 * fit + 1) > (size_t)kYCurlLds) --fit;
-    const int parts = (TR + fit - 1) / fit;
-    const int trp = (TR + parts - 1) / parts;
-    if (fp32) {
-        if (trp == 1) go_ybwd_curl<float, 1, kYCurlWaves32>(a, parts, st);
-        else go_ybwd_curl<float, 2, kYCurlWaves32>(a, parts, st);
-    } else {
-        if (trp == 1) go_ybwd_curl<double, 1, kYCurlWaves64>(a, parts, st);
-        else go_ybwd_curl<double, 2, kYCurlWaves64>(a, parts, st);
-    }
+    if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * esz * 17 > (size_t)kYCurlLds) return false;
+    const int parts = (a.rows_m + 15) / 16;
+    if (fp32) go_ybwd_curl<float, 1, kYCurlWaves>(a, parts, st);
+    else go_ybwd_curl<double, 1, kYCurlWaves>(a, parts, st);
     return true;
 }
 
