@@ -579,20 +579,24 @@ __global__ void __launch_bounds__(64 * W) k_yfwd_mfma(YArgs a) {
     // loads are unconditional, so that the compiler counts the loads in flight exactly and no
     // select waits for them: a row l' >= rows reads row rows - 1 (its table rows are zero) and a
     // column q >= NQ reads column 0 (its outputs are not written)
-    size_t o[CT];
+    // byte offsets within the field (a field is < 4 GB): loads are the uniform field base plus a
+    // 32-bit row offset plus this lane's column offset
+    const char *Yb = reinterpret_cast<const char *>(Y);
+    const unsigned rowb = (unsigned)NQ * 2u * (unsigned)sizeof(R);
+    unsigned ob[CT];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) o[c] = (size_t)(oka[c] ? qa[c] : 0) * 2 + pa;
+    for (int c = 0; c < CT; ++c) ob[c] = ((unsigned)(oka[c] ? qa[c] : 0) * 2u + (unsigned)pa) * (unsigned)sizeof(R);
     struct Buf {
         R y1[CT], y2[CT];
     };
     auto load = [&](Buf &b, int s) {
         int lp = 4 * s + k;
         lp = lp < rows ? lp : rows - 1;
-        const R *r1 = Y + (size_t)lp * NQ * 2, *r2 = Y + (size_t)(NY2 - lp) * NQ * 2;
+        const unsigned r1 = (unsigned)lp * rowb, r2 = (unsigned)(NY2 - lp) * rowb;
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            b.y1[c] = r1[o[c]];
-            b.y2[c] = r2[o[c]];
+            b.y1[c] = *reinterpret_cast<const R *>(Yb + (r1 + ob[c]));
+            b.y2[c] = *reinterpret_cast<const R *>(Yb + (r2 + ob[c]));
         }
     };
     auto compute = [&](const Buf &b, int s) {
