@@ -74,9 +74,10 @@ __device__ inline void fill_lds(R *dst, const double *src, int n, Idx idx) {
         }
     }
 }
-// fused backward: one row tile (16 l') per wavefront and 4-wavefront blocks in both precisions
-// (FP32: 112 VGPRs, 4 waves per SIMD; 50.9 against 53.6 us with two tiles on 8-wavefront
-// blocks at Re_tau=180); the loads are double-buffered; LDS budget of its two tables
+// fused backward: 4-wavefront blocks; one row tile (16 l') per wavefront (FP32 at Re_tau=180:
+// 112 VGPRs, 4 waves per SIMD, 50.9 against 53.6 us with two tiles on 8-wavefront blocks), two
+// for long FP32 columns (see launch_ybwd_curl); the loads are double-buffered; LDS budget of its
+// two tables
 constexpr int kYCurlWaves = 4;
 constexpr int kYCurlLds = 64 * 1024;
 // forward: waves per block (the parts' tables are shared by all of a block's column groups)
@@ -752,9 +753,15 @@ bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
     y_shape(a);
     const size_t esz = fp32 ? 4 : 8;
     if (!a.coefd || a.nj > a.NY2 || (size_t)2 * a.njp * esz * 17 > (size_t)kYCurlLds) return false;
-    const int parts = (a.rows_m + 15) / 16;
-    if (fp32) go_ybwd_curl<float, 1, kYCurlWaves>(a, parts, st);
-    else go_ybwd_curl<double, 1, kYCurlWaves>(a, parts, st);
+    const int TR = (a.rows_m + 15) / 16;
+    // FP32 columns longer than 128 entries: two row tiles per wavefront (Re_tau=395, 200-entry
+    // columns: 243 against 287 us with one); otherwise one (Re_tau=180: 49 against 53 us)
+    if (fp32 && a.njp > 128 && (size_t)2 * a.njp * esz * 33 <= (size_t)kYCurlLds) {
+        go_ybwd_curl<float, 2, kYCurlWaves>(a, (TR + 1) / 2, st);
+        return true;
+    }
+    if (fp32) go_ybwd_curl<float, 1, kYCurlWaves>(a, TR, st);
+    else go_ybwd_curl<double, 1, kYCurlWaves>(a, TR, st);
     return true;
 }
 
