@@ -308,12 +308,13 @@ template <typename R, int H, int S> __device__ inline void pair_dif(cpx<R> (&v)[
 }
 
 // Two-pass FFT of N = N1 N2 (N1, N2 even) on lane pairs: row thread t (pass A: t < N1, pass B:
-// t < N2) is the pair (t, h).  Pass A input: DIT layout x[m] = line[t + N1 (2m + h)] or, with
-// AIN_HALVES, line[t + N1 (H2 h + m)].  Pass B output: x[k] = X[t + N2 (H1 h + k)], which is the
-// halves input layout of the transposed plan.  x has max(N1, N2) / 2 entries.
-template <typename R, int N1, int N2, int S, bool AIN_HALVES, int M, typename Sync>
+// t < N2) is the pair (t, h); the pass-A rows sit in T at a pitch of PA entries.  Pass A input:
+// DIT layout x[m] = line[t + N1 (2m + h)] or, with AIN_HALVES, line[t + N1 (H2 h + m)].  Pass B
+// output: x[k] = X[t + N2 (H1 h + k)], which is the halves input layout of the transposed plan.
+// x has max(N1, N2) / 2 entries.
+template <typename R, int N1, int N2, int S, bool AIN_HALVES, int PA = N2 + 1, int M, typename Sync>
 __device__ __attribute__((always_inline)) inline void two_pass_pair(cpx<R> (&x)[M], int t, int h, const cpx<R> *tw, cpx<R> *T, Sync sync) {
-    constexpr int N = N1 * N2, PA = N2 + 1, H1 = N1 / 2, H2 = N2 / 2;
+    constexpr int N = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
     static_assert(N1 % 2 == 0 && N2 % 2 == 0, "pair FFT needs even factors");
     if (t < N1) {
         cpx<R> a[H2];
@@ -343,6 +344,60 @@ __device__ __attribute__((always_inline)) inline void two_pass_pair(cpx<R> (&x)[
         sfor<0, H1>([&](auto m) { b[m] = T[(2 * decltype(m)::value + h) * PA + t]; });
         pair_dit<R, H1, S>(b, h);
         sfor<0, H1>([&](auto k) { x[k] = b[k]; });
+    }
+    sync();
+}
+
+// Two independent transforms of the same plan side by side (x0 through T0, x1 through T1): each
+// pass runs both, so their arithmetic interleaves and the pair shares one barrier per pass and
+// one root-table read per element.  Row pitch PA (entries) of the pass-A output.
+template <typename R, int N1, int N2, int S, bool AIN_HALVES, int PA, int M, typename Sync>
+__device__ __attribute__((always_inline)) inline void two_pass_pair2(cpx<R> (&x0)[M], cpx<R> (&x1)[M], int t, int h,
+                                                                     const cpx<R> *tw, cpx<R> *T0, cpx<R> *T1,
+                                                                     Sync sync) {
+    constexpr int N = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
+    static_assert(N1 % 2 == 0 && N2 % 2 == 0, "pair FFT needs even factors");
+    if (t < N1) {
+        cpx<R> a0[H2], a1[H2];
+        sfor<0, H2>([&](auto j) {
+            a0[j] = x0[j];
+            a1[j] = x1[j];
+        });
+        int k2base, k2step;
+        if constexpr (AIN_HALVES) {
+            pair_dif<R, H2, S>(a0, h);
+            pair_dif<R, H2, S>(a1, h);
+            k2base = h;
+            k2step = 2;
+        } else {
+            pair_dit<R, H2, S>(a0, h);
+            pair_dit<R, H2, S>(a1, h);
+            k2base = H2 * h;
+            k2step = 1;
+        }
+        sfor<0, H2>([&](auto k) {
+            constexpr int K = decltype(k)::value;
+            const int k2 = k2base + k2step * K;
+            cpx<R> w = tw[(t * k2) % N];
+            if (S > 0) w.y = -w.y;
+            const cpx<R> v0 = a0[K], v1 = a1[K];
+            T0[t * PA + k2] = {v0.x * w.x - v0.y * w.y, v0.x * w.y + v0.y * w.x};
+            T1[t * PA + k2] = {v1.x * w.x - v1.y * w.y, v1.x * w.y + v1.y * w.x};
+        });
+    }
+    sync();
+    if (t < N2) {
+        cpx<R> b0[H1], b1[H1];
+        sfor<0, H1>([&](auto m) {
+            b0[m] = T0[(2 * decltype(m)::value + h) * PA + t];
+            b1[m] = T1[(2 * decltype(m)::value + h) * PA + t];
+        });
+        pair_dit<R, H1, S>(b0, h);
+        pair_dit<R, H1, S>(b1, h);
+        sfor<0, H1>([&](auto k) {
+            x0[k] = b0[k];
+            x1[k] = b1[k];
+        });
     }
     sync();
 }

@@ -146,23 +146,36 @@ template <int N1, int N2> struct PGeo {
 // points per block: 8 in FP64, 16 in FP32, so that each output row segment [l][kx][iz..] is 128 B
 template <typename R> constexpr int kXWaves = sizeof(R) == 8 ? 4 : 8;
 
+// Row pitch (entries) of a lane-pair transpose with n entries per row: n itself when n = 2 or 6
+// (mod 8), so that the pass-A writes of each 8-lane group hit distinct 16-byte bank groups
+// (14: pitch 14), else n + 1 (24: 25; 16: 17).  Entries of one x-product transpose buffer.
+constexpr int xpitch(int n) { return (n % 8 == 2 || n % 8 == 6) ? n : n + 1; }
+template <int N1, int N2> constexpr int xbuf_entries() {
+    constexpr int a = N1 * xpitch(N2), b = N2 * xpitch(N1), n = N1 * N2;
+    return a > b ? (a > n ? a : n) : (b > n ? b : n);
+}
+
 template <typename R, int N1, int N2>
 __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     using Gm = PGeo<N1, N2>;
     constexpr int W = kXWaves<R>;
-    constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, LB = Gm::LB, H1 = N1 / 2, H2 = N2 / 2;
+    constexpr int N = Gm::N, MH = Gm::MH, G = Gm::G, NG = Gm::NG, H1 = N1 / 2, H2 = N2 / 2;
     constexpr int PB = W * NG; // points per block
     constexpr int KP = (N / 2 + G) / G; // retained kx <= K <= N/2: entries k = r + G i, i < KP
+    // two transpose buffers per point: the two forward transforms run side by side, each pass
+    // once for both
+    constexpr int PI = xpitch(N2), PF = xpitch(N1), XB = xbuf_entries<N1, N2>();
     __shared__ cpx<R> tw[N];
-    __shared__ cpx<R> buf[PB][LB];
+    __shared__ cpx<R> buf[PB][2 * XB];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / G, r = lane - g * G, t = r >> 1, h = r & 1;
     const int gb = wv * NG + g;
     const int np = a.NL * a.NZ2, K = a.nkx - 1;
     const int p0 = blockIdx.x * PB, p = p0 + gb;
     const bool valid = p < np;
-    cpx<R> *B = buf[gb];
+    cpx<R> *B0 = buf[gb], *B1 = buf[gb] + XB;
     // each line's loads are branch-free, so they are all in flight before the first use; the
-    // first line's loads also overlap the root table's
+    // first line's loads also overlap the root table's.  The lines go one by one: issuing all
+    // three lines' loads first costs some 50 more VGPRs in FP64
     cpx<R> ph[3][MH];
     const unsigned prow = (unsigned)(valid ? p : 0) * (unsigned)a.nkx;
     sfor<0, 3>([&](auto line) {
@@ -180,7 +193,7 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
             load_roots<R, N>(tw, a.tw);
             __syncthreads();
         }
-        two_pass_pair<R, N1, N2, +1, false>(ph[LN], t, h, tw, B, lsync);
+        two_pass_pair<R, N1, N2, +1, false, PI>(ph[LN], t, h, tw, LN == 1 ? B1 : B0, lsync);
     });
     // u x w at the points n = t + N2 (H1 h + k) held by this lane
     cpx<R> F[MH], Fz[MH];
@@ -196,46 +209,26 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
         });
     }
     const R half = R(0.5) / R(N);
-    // the block's retained coefficients go out as rows [l][kx][iz0 .. iz0 + PB) of the line-major
-    // layout: B[k] and B[nkx + k] of each point, gathered after a block barrier
-    // thread i writes point j = i mod PB of rows i / PB, i / PB + 64 W / PB, ...
-    const int wj = threadIdx.x % PB, wr = threadIdx.x / PB, pw = p0 + wj;
-    const int wl = pw / a.NZ2;
-    const size_t wbase = ((size_t)wl * a.nkx) * a.NZ2 + (pw - wl * a.NZ2);
-    auto write_rows = [&](int nfo, int fo0) {
-        __syncthreads();
-        if (pw < np)
-            for (int fo = 0; fo < nfo; ++fo) {
-                cpx<R> *out = reinterpret_cast<cpx<R> *>(a.fout[fo0 + fo]) + wbase;
-                for (int k = wr; k < a.nkx; k += 64 * W / PB) out[(size_t)k * a.NZ2] = buf[wj][fo * a.nkx + k];
-            }
-        __syncthreads();
-    };
-    // forward of (Nx + i Ny): transposed plan, halves input layout already in place; the separated
-    // coefficients wait in registers while the Nz transform reuses the buffer
-    two_pass_pair<R, N2, N1, -1, true>(F, t, h, tw, B, lsync);
-    if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = F[k]; });
+    // forwards of (Nx + i Ny) and of Nz side by side: transposed plan, halves input layout
+    // already in place
+    two_pass_pair2<R, N2, N1, -1, true, PF>(F, Fz, t, h, tw, B0, B1, lsync);
+    if (t < N1)
+        sfor<0, H2>([&](auto k) {
+            B0[t + N1 * (H2 * h + decltype(k)::value)] = F[k];
+            B1[t + N1 * (H2 * h + decltype(k)::value)] = Fz[k];
+        });
     lsync();
     cpx<R> sx[KP], sy[KP], sz[KP];
     sfor<0, KP>([&](auto i) {
         const int k = r + G * decltype(i)::value;
         if (k <= K) {
-            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
+            const cpx<R> f = B0[k], gm = conjc(B0[k == 0 ? 0 : N - k]);
             const cpx<R> s = f + gm, d = f - gm;
             sx[i] = {half * s.x, half * s.y};
             sy[i] = {half * d.y, -half * d.x};
-        }
-    });
-    lsync();
-    two_pass_pair<R, N2, N1, -1, true>(Fz, t, h, tw, B, lsync);
-    if (t < N1) sfor<0, H2>([&](auto k) { B[t + N1 * (H2 * h + decltype(k)::value)] = Fz[k]; });
-    lsync();
-    sfor<0, KP>([&](auto i) {
-        const int k = r + G * decltype(i)::value;
-        if (k <= K) {
-            const cpx<R> f = B[k], gm = conjc(B[k == 0 ? 0 : N - k]);
-            const cpx<R> s = f + gm;
-            sz[i] = {half * s.x, half * s.y};
+            const cpx<R> fz = B1[k], gz = conjc(B1[k == 0 ? 0 : N - k]);
+            const cpx<R> s2 = fz + gz;
+            sz[i] = {half * s2.x, half * s2.y};
         }
     });
     // all of the group's reads precede its writes (one wavefront, LDS in program order)
@@ -243,12 +236,24 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     sfor<0, KP>([&](auto i) {
         const int k = r + G * decltype(i)::value;
         if (k <= K) {
-            B[k] = sx[i];
-            B[a.nkx + k] = sy[i];
-            B[2 * a.nkx + k] = sz[i];
+            B0[k] = sx[i];
+            B0[a.nkx + k] = sy[i];
+            B0[2 * a.nkx + k] = sz[i];
         }
     });
-    write_rows(3, 0);
+    // the block's retained coefficients go out as rows [l][kx][iz0 .. iz0 + PB) of the line-major
+    // layout, gathered after a block barrier: thread i writes point j = i mod PB of rows i / PB,
+    // i / PB + 64 W / PB, ...
+    __syncthreads();
+    const int wj = threadIdx.x % PB, wr = threadIdx.x / PB, pw = p0 + wj;
+    if (pw < np) {
+        const int wl = pw / a.NZ2;
+        const size_t wbase = ((size_t)wl * a.nkx) * a.NZ2 + (pw - wl * a.NZ2);
+        for (int fo = 0; fo < 3; ++fo) {
+            cpx<R> *out = reinterpret_cast<cpx<R> *>(a.fout[fo]) + wbase;
+            for (int k = wr; k < a.nkx; k += 64 * W / PB) out[(size_t)k * a.NZ2] = buf[wj][fo * a.nkx + k];
+        }
+    }
 }
 
 // ------------------------------------------- single-field x stages (physical) --
@@ -504,7 +509,7 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
 template <typename R, int N1, int N2> static void go_xprod(const XArgs &a, hipStream_t st) {
     if constexpr (N1 % 2 == 0 && N2 % 2 == 0) {
         // the three retained rows of a point are staged in its transpose buffer
-        if (a.pair && 3 * a.nkx <= PGeo<N1, N2>::LB) {
+        if (a.pair && 3 * a.nkx <= 2 * xbuf_entries<N1, N2>()) {
             constexpr int PB = kXWaves<R> * PGeo<N1, N2>::NG;
             hipLaunchKernelGGL((k_xprod_pair<R, N1, N2>), dim3((a.NL * a.NZ2 + PB - 1) / PB), dim3(64 * kXWaves<R>),
                                0, st, a);
