@@ -7,6 +7,7 @@
 // the LDS Stockham kernels of transforms.hip.
 #include "internal.hpp"
 #include "regfft.hpp"
+#include <type_traits>
 
 namespace cdns {
 
@@ -174,11 +175,11 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
     const bool valid = p < np;
     cpx<R> *B0 = buf[gb], *B1 = buf[gb] + XB;
     // each line's loads are branch-free, so they are all in flight before the first use; the
-    // first line's loads also overlap the root table's.  The lines go one by one: issuing all
-    // three lines' loads first costs some 50 more VGPRs in FP64
+    // first line's loads also overlap the root table's.  Issuing all three lines' loads before
+    // the first transform costs some 50 more VGPRs in FP64
     cpx<R> ph[3][MH];
     const unsigned prow = (unsigned)(valid ? p : 0) * (unsigned)a.nkx;
-    sfor<0, 3>([&](auto line) {
+    auto load_line = [&](auto line) {
         constexpr int LN = decltype(line)::value;
         const cpx<R> *A = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN]);
         const cpx<R> *Bv = reinterpret_cast<const cpx<R> *>(a.fin[2 * LN + 1]);
@@ -189,12 +190,28 @@ __global__ void __launch_bounds__(64 * kXWaves<R>) k_xprod_pair(XArgs a) {
             else
                 ph[LN][M] = cpx<R>{R(0), R(0)};
         });
-        if constexpr (LN == 0) {
-            load_roots<R, N>(tw, a.tw);
-            __syncthreads();
-        }
-        two_pass_pair<R, N1, N2, +1, false, PI>(ph[LN], t, h, tw, LN == 1 ? B1 : B0, lsync);
-    });
+    };
+    if constexpr (sizeof(R) == 8 && MH <= 7) {
+        // FP64 rows of up to 14 points: lines 0 and 1 side by side, then line 2 (156 VGPRs for
+        // 196-point lines, still 3 waves per SIMD; the same in FP32 costs two waves per SIMD)
+        load_line(std::integral_constant<int, 0>{});
+        load_line(std::integral_constant<int, 1>{});
+        load_roots<R, N>(tw, a.tw);
+        __syncthreads();
+        two_pass_pair2<R, N1, N2, +1, false, PI>(ph[0], ph[1], t, h, tw, B0, B1, lsync);
+        load_line(std::integral_constant<int, 2>{});
+        two_pass_pair<R, N1, N2, +1, false, PI>(ph[2], t, h, tw, B0, lsync);
+    } else {
+        sfor<0, 3>([&](auto line) {
+            constexpr int LN = decltype(line)::value;
+            load_line(line);
+            if constexpr (LN == 0) {
+                load_roots<R, N>(tw, a.tw);
+                __syncthreads();
+            }
+            two_pass_pair<R, N1, N2, +1, false, PI>(ph[LN], t, h, tw, LN == 1 ? B1 : B0, lsync);
+        });
+    }
     // u x w at the points n = t + N2 (H1 h + k) held by this lane
     cpx<R> F[MH], Fz[MH];
     sfor<0, MH>([&](auto j) {
