@@ -1770,9 +1770,9 @@ int cdns_set_state(cdns_ctx *c, int field, const double *data, int where) {
         state_planes(c, field, pl, ncomp, nrow, JS);
         if (c->nl32 && (field == 2 || field == 3)) {
             // single-precision planes: staged through FP64 planes
-            zd *st[3] = {c->scr[0], c->scr[1], c->scr[2]};
-            ref_to_planes(c, data, ncomp, ncomp, nrow, st, where, JS);
-            for (int k = 0; k < 3; ++k) launch_narrow(st[k], pl[k], c->g.S, c->st);
+            zd *stage[3] = {c->scr[0], c->scr[1], c->scr[2]};
+            ref_to_planes(c, data, ncomp, ncomp, nrow, stage, where, JS);
+            for (int k = 0; k < 3; ++k) launch_narrow(stage[k], pl[k], c->g.S, c->st);
         } else {
             ref_to_planes(c, data, ncomp, ncomp, nrow, pl, where, JS);
         }
