@@ -377,6 +377,32 @@ template <typename R, int N1, int N2> struct ZTile {
 };
 template <typename R, int N1, int N2> constexpr int kZThreads = 64 * ZTile<R, N1, N2>::WAVES;
 
+// Tile store of a z backward block: lds[iz * PITCH + li] -> Z[iz * nkx + li], li < nl.  A full tile
+// (nl = KB, all blocks but a plane's last) gives each thread one li and a fixed row stride, addressed
+// as 32-bit offsets from the block's base: no division and no 64-bit address arithmetic per element.
+template <typename R, int N, int KB, int PITCH, int NT>
+__device__ inline void store_ztile(cpx<R> *Z, const cpx<R> *lds, int nl, int nkx) {
+    static_assert(NT % KB == 0, "whole rows per block");
+    if (nl == KB) {
+        constexpr int RS = NT / KB;
+        const int li = threadIdx.x % KB, r0 = threadIdx.x / KB;
+        char *zb = reinterpret_cast<char *>(Z);
+        const unsigned es = (unsigned)sizeof(cpx<R>), os = (unsigned)(RS * nkx) * es;
+        const unsigned o = (unsigned)(r0 * nkx + li) * es;
+        sfor<0, (N + RS - 1) / RS>([&](auto i) {
+            constexpr int I = decltype(i)::value;
+            const int iz = r0 + I * RS;
+            if (I * RS + RS <= N || iz < N)
+                *reinterpret_cast<cpx<R> *>(zb + o + (unsigned)I * os) = lds[iz * PITCH + li];
+        });
+        return;
+    }
+    for (int e = threadIdx.x; e < N * nl; e += NT) {
+        const int iz = e / nl, li = e - iz * nl;
+        Z[(size_t)iz * nkx + li] = lds[iz * PITCH + li];
+    }
+}
+
 // backward: slot s of the kz axis <- retained kz, FFT over the slots, Z[l][iz][kx]
 template <typename R, int N1, int N2>
 __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
@@ -416,10 +442,7 @@ __global__ void __launch_bounds__((kZThreads<R, N1, N2>)) k_zbwd_rf(ZArgs a) {
     if (valid && t < N2) sfor<0, N1>([&](auto k1) { lds[(t + N2 * decltype(k1)::value) * PITCH + line] = x[k1]; });
     __syncthreads();
     cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx0;
-    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
-        const int iz = e / nl, li = e - iz * nl;
-        Z[(size_t)iz * a.nkx + li] = lds[iz * PITCH + li];
-    }
+    store_ztile<R, N, kZLines, PITCH, kZThreads<R, N1, N2>>(Z, lds, nl, a.nkx);
 }
 
 // forward: line (f, l, kx) over iz; keep the retained kz, x 1/NZ2.
@@ -510,16 +533,15 @@ __global__ void __launch_bounds__((kZPThreads<R, N1, N2>)) k_zbwd_pair(ZArgs a) 
     }
     load_roots<R, N>(tw, a.tw); // after the line loads: both latencies overlap
     __syncthreads();
-    two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, lds + line * LB, lsync);
+    // a plane's last block can hold fewer lines than waves x lines per wave: a wavefront whose lines
+    // all lie past the plane's end skips the transform (its passes synchronise only within the wave)
+    if (wv * NG < nl) two_pass_pair<R, N1, N2, +1, false>(x, t, h, tw, lds + line * LB, lsync);
     __syncthreads(); // every line's transposes are done: the tile may overwrite their buffers
     if (valid && t < N2)
         sfor<0, H1>([&](auto k) { lds[(t + N2 * (H1 * h + decltype(k)::value)) * PITCH + line] = x[k]; });
     __syncthreads();
     cpx<R> *Z = reinterpret_cast<cpx<R> *>(a.out) + (size_t)f * a.out_fs + (size_t)l * N * a.nkx + kx0;
-    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
-        const int iz = e / nl, li = e - iz * nl;
-        Z[(size_t)iz * a.nkx + li] = lds[iz * PITCH + li];
-    }
+    store_ztile<R, N, kZLines, PITCH, kZPThreads<R, N1, N2>>(Z, lds, nl, a.nkx);
 }
 
 // ------------------------------------------------------------------ launch --
