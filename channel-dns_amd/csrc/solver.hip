@@ -126,24 +126,39 @@ __device__ void tau_solve(const zd (&r)[NR][NE], zd (&out)[NR][NE], const double
 // (simulation.cpp:111-140) fused in MODE_STEP; the setup problems of eqsolve_pv.cpp:266-303
 // and 452-477 in MODE_INFLUENCE / MODE_BPROBLEM.
 template <int NE, int MODE, bool NL32 = false>
-__global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE == 2 ? 3 : 1))) k_solve(SolveArgs a) {
     const int lane = lane_id(), p = sp::parity(lane);
     // the column index is wavefront-uniform: made visible as such, the per-column scalars
-    // (wavenumbers, wall values, influence and sigma entries) come through scalar loads
-    const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    if (q >= a.NQ) return;
+    // (wavenumbers, wall values, influence and sigma entries) come through scalar loads.
+    // Wavefront w -> column: the columns kz = -m and kz = +m of one kx on neighbouring wavefronts
+    // of a block, then the kz = 0 columns.  The real tables (ad, and the compact influence and B
+    // responses) depend on kx^2 + kz^2 only, bit for bit (the setup's arithmetic is symmetric
+    // under kz -> -kz), so both columns of a pair read the pair's kz < 0 column (qt) and the
+    // second read of each table line is served from the cache, not from HBM.
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (w >= a.NQ) return;
+    const int nzk = a.nzk, hz = (nzk - 1) >> 1, npair = (a.NQ / nzk) * 2 * hz;
+    int kxi, kzi;
+    if (w < npair) {
+        const int pr = w >> 1, m = pr % hz + 1;
+        kxi = pr / hz;
+        kzi = (w & 1) ? hz + m : hz - m;
+    } else {
+        kxi = w - npair;
+        kzi = hz;
+    }
+    const int q = kxi * nzk + kzi, qt = kxi * nzk + (kzi < hz ? kzi : nzk - 1 - kzi);
     const int ny = a.ny;
     constexpr int mode = MODE;
-    const int kxi = q / a.nzk, kzi = q - kxi * a.nzk;
-    const bool zero = kxi == 0 && 2 * kzi == a.nzk - 1;
+    const bool zero = kxi == 0 && kzi == hz;
     const double kx = a.WAVX[kxi], kz = a.WAVZ[kzi];
-    const size_t off = (size_t)q * a.JS;
+    const size_t off = (size_t)q * a.JS, toff = (size_t)qt * a.JS;
     double WL[NE], WD[NE], WR[NE], adp[NE], adv[NE];
     sp::load_rcol<NE>(a.WL, lane, ny, WL);
     sp::load_rcol<NE>(a.WD, lane, ny, WD);
     sp::load_rcol<NE>(a.WR, lane, ny, WR);
-    sp::load_rcol<NE>(a.ad_p + off, lane, ny, adp);
-    sp::load_rcol<NE>(a.ad_v + off, lane, ny, adv);
+    sp::load_rcol<NE>(a.ad_p + toff, lane, ny, adp);
+    sp::load_rcol<NE>(a.ad_v + toff, lane, ny, adv);
 
     // ---- velocity right-hand side
     zd rv[3][NE];
@@ -319,9 +334,9 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
             zd ptop, vt0, vt1;
             zd s0, s1;
             if (compact) {
-                ptop = sp::col_at<NE>(pc[0], ny) + (a.PIr[0][off + ny] * c0 + a.PIr[1][off + ny] * c1);
-                vt0 = sp::col_at<NE>(vc[1], ny) + (a.VIv[0][off + ny] * c0 + a.VIv[1][off + ny] * c1);
-                vt1 = sp::col_at<NE>(vc[1], ny - 1) + (a.VIv[0][off + ny - 1] * c0 + a.VIv[1][off + ny - 1] * c1);
+                ptop = sp::col_at<NE>(pc[0], ny) + (a.PIr[0][toff + ny] * c0 + a.PIr[1][toff + ny] * c1);
+                vt0 = sp::col_at<NE>(vc[1], ny) + (a.VIv[0][toff + ny] * c0 + a.VIv[1][toff + ny] * c1);
+                vt1 = sp::col_at<NE>(vc[1], ny - 1) + (a.VIv[0][toff + ny - 1] * c0 + a.VIv[1][toff + ny - 1] * c1);
                 s0 = sgc[0];
                 s1 = sgc[1];
             } else {
@@ -342,19 +357,19 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
     if (a.correct && a.sigma && compact) {
         // P += c0 P0 + c1 P1 + s PB;  v += c0 v0 + c1 v1 + s vB;  (u, w) += i (kx, kz) (c0 f0 + c1 f1 + s fB)
         double r0[NE], r1[NE], rb[NE];
-        sp::load_rcol<NE>(a.PIr[0] + off, lane, ny, r0);
-        sp::load_rcol<NE>(a.PIr[1] + off, lane, ny, r1);
-        sp::load_rcol<NE>(a.PBr + off, lane, ny, rb);
+        sp::load_rcol<NE>(a.PIr[0] + toff, lane, ny, r0);
+        sp::load_rcol<NE>(a.PIr[1] + toff, lane, ny, r1);
+        sp::load_rcol<NE>(a.PBr + toff, lane, ny, rb);
 #pragma unroll
         for (int e = 0; e < NE; ++e) P[e] = pc[0][e] + ((r0[e] * c0 + r1[e] * c1) + rb[e] * sp_other);
-        sp::load_rcol<NE>(a.VIv[0] + off, lane, ny, r0);
-        sp::load_rcol<NE>(a.VIv[1] + off, lane, ny, r1);
-        sp::load_rcol<NE>(a.VBv + off, lane, ny, rb);
+        sp::load_rcol<NE>(a.VIv[0] + toff, lane, ny, r0);
+        sp::load_rcol<NE>(a.VIv[1] + toff, lane, ny, r1);
+        sp::load_rcol<NE>(a.VBv + toff, lane, ny, rb);
 #pragma unroll
         for (int e = 0; e < NE; ++e) V[1][e] = vc[1][e] + ((r0[e] * c0 + r1[e] * c1) + rb[e] * sp_own);
-        sp::load_rcol<NE>(a.VIf[0] + off, lane, ny, r0);
-        sp::load_rcol<NE>(a.VIf[1] + off, lane, ny, r1);
-        sp::load_rcol<NE>(a.VBf + off, lane, ny, rb);
+        sp::load_rcol<NE>(a.VIf[0] + toff, lane, ny, r0);
+        sp::load_rcol<NE>(a.VIf[1] + toff, lane, ny, r1);
+        sp::load_rcol<NE>(a.VBf + toff, lane, ny, rb);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const zd phi = (r0[e] * c0 + r1[e] * c1) + rb[e] * sp_other;
