@@ -44,8 +44,9 @@ def algorithmic_bytes(cfg, esz):
         "z_forward": 3 * Z * esz + 3 * Y * esz,
         "y_forward": 3 * Y * esz + 3 * S * zc,
         # u, u_prev, N, N_prev in; the influence/B responses as 9 real columns (P and v real,
-        # u = i kx phi, w = i kz phi) and 2 real ad columns; u, P out
-        "solve_step": 12 * S * zc + 9 * S * 8 + 2 * S * 8 + 4 * S * zc,
+        # u = i kx phi, w = i kz phi) and 2 real ad columns, read once for the two columns
+        # kz = -m, +m of a kx that share them (nkx (NZS + 1) distinct columns); u, P out
+        "solve_step": 12 * S * zc + 11 * nkx * (cfg["NZS"] + 1) * nj * 8 + 4 * S * zc,
     }
 
 
@@ -59,12 +60,12 @@ DEFAULT_YSTAGE = "mfma"
 # SQ_INSTS_VALU of one x-product launch on this workload (FP64; rocprofv3 --pmc) and the issue
 # peak it is set against: a SIMD is 16 lanes wide, so it issues one wave64 vector instruction
 # per 4 cycles; 1024 SIMDs at 2.4 GHz.
-MEASURED_VALU = {"x_product": 44.4e6}
+MEASURED_VALU = {"x_product": 42.6e6}
 VALU_PEAK_WAVE_INSTS = 1024 * 2.4e9 / 4
 
 MEASURED_TRAFFIC = {
     "x_product": 356.3e6,
-    "solve_step": 374.4e6,
+    "solve_step": 329.2e6,
 }
 
 
