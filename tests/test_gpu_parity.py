@@ -197,6 +197,26 @@ def test_setup_tables_walls(gpu, name, walls):
             assert np.array_equal(b, np.array([1.0, 0.0])) and maxrel(a, b) < 1e-15, t
 
 
+# The step solve reads the real tables (ad and the compact influence and B responses) of the
+# column kz = -m for both columns kz = -m, +m of one kx.  That relies on the setup making them
+# alike bit for bit: only kx^2 + kz^2 enters, so P, u, v and the wall divergences are equal and
+# w is negated exactly (the forcing i kz P changes sign and every operation after it is odd in it).
+@pytest.mark.parametrize("walls", ["dirichlet", "robin", "mixed_uw"])
+@pytest.mark.parametrize("name", ["channel_32", "stage3"])
+def test_setup_tables_mirror_kz(gpu, name, walls):
+    cfg = _with_walls(stage3_config() if name == "stage3" else CONFIGS[name], walls)
+    dev = gpu.Channel(cfg)
+    dev.setup(1.5 / cfg["DT"] * cfg["Re"])
+    # leading axis: kz index kzi = kz + NZS, mirrored by [::-1]
+    for t in ("ad_p", "ad_v", "P_infl0", "P_infl1", "P_b", "infl_div0", "infl_div1", "sigma0"):
+        a = dev.table(t)
+        assert np.all(np.isfinite(a)) and np.array_equal(a, a[::-1]), t
+    for t in ("V_infl0", "V_infl1", "V_b"):
+        a = dev.table(t)
+        assert np.array_equal(a[..., :2], a[::-1, ..., :2]), t
+        assert np.array_equal(a[..., 2], -a[::-1, ..., 2]), t
+
+
 @pytest.mark.parametrize("name,walls,fft", [("channel_32", "robin", "register"),
                                             ("channel_32", "mixed_uw", "register"),
                                             ("stage3", "robin", "register"),
