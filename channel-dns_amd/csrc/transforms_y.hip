@@ -683,8 +683,8 @@ template <typename R, int TRP, int CT> static void go_ybwd(YArgs a, int parts, i
     a.ncg = (a.NQ + cols - 1) / cols;
     hipLaunchKernelGGL((k_ybwd_mfma<R, TRP, CT>), dim3(a.parts * a.ncg * nf, 1), dim3(64 * kWaves), sm, st, a);
 }
-template <typename R, int JTP, int CT> static void go_yfwd(YArgs a, int parts, int nf, hipStream_t st) {
-    constexpr int W = kYFwdWaves;
+template <typename R, int JTP, int CT, int W = kYFwdWaves>
+static void go_yfwd(YArgs a, int parts, int nf, hipStream_t st) {
     const size_t sm = ((size_t)2 * a.KL * JTP * 16 + JTP * 16) * sizeof(R);
     const int cols = 8 * CT * W;
     a.parts = parts;
@@ -765,6 +765,20 @@ bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
     return true;
 }
 
+// compute units of the current device (first query; 256 if the query fails)
+static int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
 bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
     y_shape(a);
     const size_t esz = fp32 ? 4 : 8;
@@ -773,6 +787,19 @@ bool launch_yfwd_mfma(YArgs a, int nf, bool fp32, hipStream_t st) {
     const int fit = std::min(tiles_per_part((size_t)2 * a.KL * 16 * esz + 16 * esz, 0, 2), JT);
     const int parts = (JT + fit - 1) / fit;
     const int jtp = (JT + parts - 1) / parts;
+    // a grid of fewer 8-wavefront blocks of two column sets than the device has CUs (small meshes:
+    // 51 blocks at 64x65x64) runs one column set per wavefront in 2-wavefront blocks, eight times
+    // the blocks; each column's matrix operations are the same either way
+    if (parts * ((a.NQ + 8 * 2 * kYFwdWaves - 1) / (8 * 2 * kYFwdWaves)) * nf < device_cus()) {
+        if (fp32) {
+            if (jtp == 1) go_yfwd<float, 1, 1, 2>(a, parts, nf, st);
+            else go_yfwd<float, 2, 1, 2>(a, parts, nf, st);
+        } else {
+            if (jtp == 1) go_yfwd<double, 1, 1, 2>(a, parts, nf, st);
+            else go_yfwd<double, 2, 1, 2>(a, parts, nf, st);
+        }
+        return true;
+    }
     if (fp32) {
         if (jtp == 1) go_yfwd<float, 1, 2>(a, parts, nf, st);
         else go_yfwd<float, 2, 2>(a, parts, nf, st);
