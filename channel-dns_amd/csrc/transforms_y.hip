@@ -765,17 +765,16 @@ bool launch_ybwd_curl(YArgs a, bool fp32, hipStream_t st) {
     return true;
 }
 
-// compute units of the current device (first query; 256 if the query fails)
+// compute units of the device current at the first call (256 if the query fails); the static's
+// initialisation runs once, also when several host threads launch
 static int device_cus() {
-    static int cus = 0;
-    if (cus == 0) {
+    static const int cus = [] {
         int dev = 0, n = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-            cus = n;
-        else
-            cus = 256;
-    }
+            return n;
+        return 256;
+    }();
     return cus;
 }
 
